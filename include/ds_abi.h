@@ -1,0 +1,131 @@
+/*
+ * C-ABI of the MI355X dual simplex (libpds.so).
+ *
+ * Plain pointers and sizes only; every function returns 0 (DS_OK) or a
+ * negative ds_err and never throws.  One host thread drives one context; a
+ * context owns one HIP stream and all device workspaces.  Host-pointer
+ * arguments are copied before the call returns.
+ *
+ * What each entry replaces in the reference (file:line under its src/):
+ *   ds_load_problem   Problem(bound_type, range_type, costs, lower_range,
+ *                     upper_range, lower_bound, upper_bound, A) +
+ *                     transformToComputeForm()       common/problem.tpp:143-226
+ *   ds_init           SequentialDualSimplex::initDualSimplex
+ *                                                   cpu/sequentialDualSimplex.cpp:4-25
+ *                     (GPU analogue: CudaDualSimplex::initDualSimplex, gpu/cudaDualSimplex.cpp:20-81)
+ *   ds_presolve       BaseDualSimplex::presolve("minInfeas") common/baseDualSimplex.tpp:5-30
+ *   ds_solve          BaseDualSimplex::solve + postProcess  common/baseDualSimplex.tpp:123-180
+ *   ds_get_solution   Problem::getSolution / LPsolution    common/problem.tpp:243-259,
+ *                                                          common/LPsolution.hpp:22-44
+ *   ds_get_basis      basis_indexes (protected in the reference, baseDualSimplex.hpp:80)
+ *   ds_refactor       reFactorize                       cpu/sequentialDualSimplex.cpp:155-160
+ *   ds_btran          BTran -> solveLinSys(e_p, rho, true)  cpu/sequentialDualSimplex.cpp:89-92
+ *   ds_ftran          solveLinSys(b, x, false)          cpu/sequentialDualSimplex.cpp:28-48
+ *   ds_ftran_column   FTran(q, alpha_q)                 cpu/sequentialDualSimplex.cpp:107-110
+ *   ds_pivot_row      pivotRow -> Matrix::dotUpdate(UPDATE_T)
+ *                                                   cpu/sequentialDualSimplex.cpp:95-104,
+ *                                                   cpu/matrix.cpp:273-287; launcher spmvUpdateInc,
+ *                                                   gpu/utillities.hpp:130-148
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ds_ctx ds_ctx;
+
+/* Bound / row types, order of the reference's BoundaryType (common/types.hpp:48-56). */
+enum { DS_FIXED = 0, DS_FREE = 1, DS_BOXED = 2, DS_UPPER = 3, DS_LOWER = 4 };
+
+/* Termination status; the reference's messages are kept by ds_get_solution. */
+enum {
+  DS_OPTIMAL = 0,            /* "optimal solution" */
+  DS_PRIMAL_INFEASIBLE = 1,  /* "dual unbounded" */
+  DS_DUAL_INFEASIBLE = 2,    /* "dual infeasible" */
+  DS_ITERATION_LIMIT = 3,
+  DS_NOT_SOLVED = 4
+};
+
+/* Error codes. */
+enum {
+  DS_OK = 0,
+  DS_EINVAL = -1,
+  DS_EHIP = -2,
+  DS_ESINGULAR = -3,
+  DS_EUNSUPPORTED = -4,
+  DS_ESTATE = -5,
+  DS_EINTERNAL = -6
+};
+
+typedef struct {
+  int refactor_freq;      /* REFACT_FREQ (reference default 150) */
+  int max_iterations;     /* 0 = no cap (reference behaviour) */
+  int ratio_test;         /* 0 = textbook min-ratio (reference), 1 = Harris two-pass */
+  int perturb;            /* 1 = reference perturbation rules, 0 = never perturb */
+  uint64_t perturb_seed;  /* seed of the cost perturbation stream */
+  int chunk;              /* iterations queued per host synchronisation (>= 1) */
+  int verbose;
+} ds_params;
+
+typedef struct {
+  int status;
+  int solved;
+  int iterations;         /* phase 1 + phase 2, the reference's shared counter */
+  int phase1_iterations;  /* iterations spent in the initial phase 1 */
+  int perturbed;
+  int refactors;
+  int restarts;
+  double Z;               /* objective (offset removed) */
+  double solve_ms;        /* device-resident solve wall time (phase 1 + phase 2) */
+  double refactor_ms;     /* part of solve_ms spent in refactorization */
+} ds_result;
+
+void ds_default_params(ds_params* p);
+
+int ds_create(int device, ds_ctx** out);
+int ds_destroy(ds_ctx* ctx);
+const char* ds_last_error(ds_ctx* ctx);
+
+/* A is m x n (structurals only) in CSC; infinite bounds as +-inf or |v|>1e100. */
+int ds_load_problem(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                    const double* costs, const double* lower_bound, const double* upper_bound,
+                    const int* bound_type, const double* lower_range, const double* upper_range,
+                    const int* range_type);
+int ds_set_params(ds_ctx* ctx, const ds_params* p);
+int ds_init(ds_ctx* ctx);
+int ds_presolve(ds_ctx* ctx, const char* method, int* phase1_status); /* 0 Solved, 1 DualInfeas */
+int ds_solve(ds_ctx* ctx);
+int ds_get_result(ds_ctx* ctx, ds_result* r);
+int ds_get_solution(ds_ctx* ctx, double* Z, int* solved, char* message, int message_len, double* x /* n+m */);
+int ds_get_basis(ds_ctx* ctx, int* basis /* m */);
+
+/* Whole solve in one call: load + init + presolve("minInfeas") + solve. */
+int ds_solve_lp(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                const double* costs, const double* lower_bound, const double* upper_bound,
+                const int* bound_type, const double* lower_range, const double* upper_range,
+                const int* range_type, const ds_params* p, ds_result* r, double* x, int* basis);
+
+/* Hook-level entries on the context's current basis (host buffers, synchronous). */
+int ds_set_basis(ds_ctx* ctx, const int* basis /* m variable ids; nonbasic set = ascending complement */);
+int ds_refactor(ds_ctx* ctx);
+int ds_btran(ds_ctx* ctx, int p_idx, double* rho /* m, by row */);
+int ds_ftran(ds_ctx* ctx, const double* b /* m, by row */, double* x /* m, by position */);
+int ds_ftran_column(ds_ctx* ctx, int j, double* x /* m */);
+int ds_pivot_row(ds_ctx* ctx, const double* rho /* m */, double* alpha /* n, by nonbasic position */);
+int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis /* n */);
+
+/* Timing of the pivot-row SpMV on the context's current basis and a given rho:
+ * `reps` launches, HIP events on the context stream; returns the mean ms per
+ * launch in *ms and the algorithmic bytes of one launch in *bytes. */
+int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, double* bytes);
+
+/* Trailing-update tile check: C(64x64, row-major) -= P(:, :nb) * B(nb x 64), where
+ * P is 64x64 row-major (columns >= nb ignored) and B is nb x 64 row-major; the
+ * FP64 MFMA kernel when use_mfma, else the plain FMA kernel. */
+int ds_debug_gemm_tile(double* C, const double* P, const double* B, int nb, int use_mfma);
+
+#ifdef __cplusplus
+}
+#endif

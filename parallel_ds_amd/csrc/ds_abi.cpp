@@ -1,0 +1,251 @@
+// extern "C" boundary of libpds.so: argument checks, exception-to-error-code
+// translation, and the host/device copies of the hook-level entries.
+#include <cstring>
+#include <string>
+
+#include "ds_abi.h"
+#include "ds_solver.hpp"
+
+struct ds_ctx {
+  pds::Solver* solver = nullptr;
+  std::string err;
+};
+
+namespace pds {
+void launch_gemm_tile_check(hipStream_t s, double* C, const double* P, const double* B, int nb, int use_mfma);
+}
+
+namespace {
+
+template <typename F>
+int guard(ds_ctx* ctx, F&& f) {
+  if (!ctx || !ctx->solver) return DS_EINVAL;
+  try {
+    f(*ctx->solver);
+    return DS_OK;
+  } catch (const pds::HipError& e) {
+    ctx->err = e.what();
+    return DS_EHIP;
+  } catch (const pds::SingularError& e) {
+    ctx->err = e.what();
+    return DS_ESINGULAR;
+  } catch (const pds::Unsupported& e) {
+    ctx->err = e.what();
+    return DS_EUNSUPPORTED;
+  } catch (const std::invalid_argument& e) {
+    ctx->err = e.what();
+    return DS_EINVAL;
+  } catch (const std::logic_error& e) {
+    ctx->err = e.what();
+    return DS_ESTATE;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return DS_EINTERNAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ds_default_params(ds_params* p) {
+  if (!p) return;
+  p->refactor_freq = 150;
+  p->max_iterations = 0;
+  p->ratio_test = 0;
+  p->perturb = 1;
+  p->perturb_seed = 0x5eedULL;
+  p->chunk = 32;
+  p->verbose = 0;
+}
+
+int ds_create(int device, ds_ctx** out) {
+  if (!out) return DS_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return DS_EHIP;
+  if (device < 0 || device >= ndev) return DS_EINVAL;
+  ds_ctx* c = new ds_ctx();
+  try {
+    c->solver = new pds::Solver(device);
+  } catch (const std::exception& e) {
+    delete c;
+    return DS_EHIP;
+  }
+  *out = c;
+  return DS_OK;
+}
+
+int ds_destroy(ds_ctx* ctx) {
+  if (!ctx) return DS_EINVAL;
+  delete ctx->solver;
+  delete ctx;
+  return DS_OK;
+}
+
+const char* ds_last_error(ds_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ds_load_problem(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                    const double* costs, const double* lower_bound, const double* upper_bound,
+                    const int* bound_type, const double* lower_range, const double* upper_range,
+                    const int* range_type) {
+  if (m <= 0 || n <= 0 || !col_ptr || !costs || !lower_bound || !upper_bound || !bound_type || !lower_range ||
+      !upper_range || !range_type)
+    return DS_EINVAL;
+  if (col_ptr[n] > 0 && (!row_idx || !val)) return DS_EINVAL;
+  // Problem::checkConstraints (problem.tpp:230-239): more rows than columns is rejected.
+  if (m > n + m) return DS_EINVAL;
+  for (int j = 0; j < n; ++j)
+    if (col_ptr[j + 1] < col_ptr[j]) return DS_EINVAL;
+  for (int e = 0; e < col_ptr[n]; ++e)
+    if (row_idx[e] < 0 || row_idx[e] >= m) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) {
+    pds::HostModel hm;
+    hm.build(m, n, col_ptr, row_idx, val, costs, lower_bound, upper_bound, bound_type, lower_range, upper_range,
+             range_type);
+    s.load(hm);
+  });
+}
+
+int ds_set_params(ds_ctx* ctx, const ds_params* p) {
+  if (!p) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.set_params(*p); });
+}
+
+int ds_init(ds_ctx* ctx) {
+  return guard(ctx, [&](pds::Solver& s) { s.initDualSimplex(); });
+}
+
+int ds_presolve(ds_ctx* ctx, const char* method, int* phase1_status) {
+  if (!method || std::strcmp(method, "minInfeas") != 0) return DS_EINVAL;  // stringToPreSolverMethod
+  return guard(ctx, [&](pds::Solver& s) {
+    int st = s.presolve();
+    if (phase1_status) *phase1_status = st;
+  });
+}
+
+int ds_solve(ds_ctx* ctx) {
+  return guard(ctx, [&](pds::Solver& s) { s.solve(); });
+}
+
+int ds_get_result(ds_ctx* ctx, ds_result* r) {
+  if (!r) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.result(*r); });
+}
+
+int ds_get_solution(ds_ctx* ctx, double* Z, int* solved, char* message, int message_len, double* x) {
+  return guard(ctx, [&](pds::Solver& s) {
+    double z;
+    int sv;
+    std::string msg;
+    std::vector<double> xv;
+    s.solution(z, sv, msg, xv);
+    if (Z) *Z = z;
+    if (solved) *solved = sv;
+    if (message && message_len > 0) {
+      std::strncpy(message, msg.c_str(), message_len - 1);
+      message[message_len - 1] = 0;
+    }
+    if (x) std::memcpy(x, xv.data(), sizeof(double) * xv.size());
+  });
+}
+
+int ds_get_basis(ds_ctx* ctx, int* basis) {
+  if (!basis) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) {
+    auto b = s.basis_host();
+    std::memcpy(basis, b.data(), sizeof(int) * b.size());
+  });
+}
+
+int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis) {
+  if (!nonbasis) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) {
+    auto b = s.nonbasis_host();
+    std::memcpy(nonbasis, b.data(), sizeof(int) * b.size());
+  });
+}
+
+int ds_solve_lp(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                const double* costs, const double* lower_bound, const double* upper_bound, const int* bound_type,
+                const double* lower_range, const double* upper_range, const int* range_type, const ds_params* p,
+                ds_result* r, double* x, int* basis) {
+  int rc = ds_load_problem(ctx, m, n, col_ptr, row_idx, val, costs, lower_bound, upper_bound, bound_type,
+                           lower_range, upper_range, range_type);
+  if (rc) return rc;
+  if (p && (rc = ds_set_params(ctx, p))) return rc;
+  if ((rc = ds_init(ctx))) return rc;
+  int st = 0;
+  if ((rc = ds_presolve(ctx, "minInfeas", &st))) return rc;
+  if (st == 0 && (rc = ds_solve(ctx))) return rc;
+  if (r && (rc = ds_get_result(ctx, r))) return rc;
+  if (x && (rc = ds_get_solution(ctx, nullptr, nullptr, nullptr, 0, x))) return rc;
+  if (basis && (rc = ds_get_basis(ctx, basis))) return rc;
+  return DS_OK;
+}
+
+int ds_set_basis(ds_ctx* ctx, const int* basis) {
+  if (!basis) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.set_basis(basis); });
+}
+
+int ds_refactor(ds_ctx* ctx) {
+  return guard(ctx, [&](pds::Solver& s) { s.refactor(); });
+}
+
+int ds_btran(ds_ctx* ctx, int p_idx, double* rho) {
+  if (!rho) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.btran(p_idx, rho); });
+}
+
+int ds_ftran(ds_ctx* ctx, const double* b, double* x) {
+  if (!b || !x) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.ftran(b, x); });
+}
+
+int ds_ftran_column(ds_ctx* ctx, int j, double* x) {
+  if (!x) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.ftran_column(j, x); });
+}
+
+int ds_pivot_row(ds_ctx* ctx, const double* rho, double* alpha) {
+  if (!rho || !alpha) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.pivot_row(rho, alpha); });
+}
+
+int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, double* bytes) {
+  if (!rho || !ms || !bytes || reps <= 0) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.time_pivot_row(rho, reps, *ms, *bytes); });
+}
+
+int ds_debug_gemm_tile(double* C, const double* P, const double* B, int nb, int use_mfma) {
+  if (!C || !P || !B || nb <= 0 || nb > pds::GJ_NB) return DS_EINVAL;
+  double *dC = nullptr, *dP = nullptr, *dB = nullptr;
+  const size_t sc = 64 * 64 * sizeof(double), sp = 64 * pds::GJ_NB * sizeof(double);
+  if (hipMalloc((void**)&dC, sc) != hipSuccess) return DS_EHIP;
+  if (hipMalloc((void**)&dP, sp) != hipSuccess) {
+    (void)hipFree(dC);
+    return DS_EHIP;
+  }
+  if (hipMalloc((void**)&dB, sp) != hipSuccess) {
+    (void)hipFree(dC);
+    (void)hipFree(dP);
+    return DS_EHIP;
+  }
+  int rc = DS_OK;
+  if (hipMemcpy(dC, C, sc, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dP, P, sp, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dB, B, nb * 64 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = DS_EHIP;
+  } else {
+    pds::launch_gemm_tile_check(nullptr, dC, dP, dB, nb, use_mfma);
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(C, dC, sc, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = DS_EHIP;
+  }
+  (void)hipFree(dC);
+  (void)hipFree(dP);
+  (void)hipFree(dB);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1086 @@
+// gfx950 kernels of one dual simplex iteration and of its (re)initialisation.
+//
+// Every kernel reads the device status word at entry and returns when an
+// earlier kernel of the queued chunk raised an event; the host then handles
+// the event and resumes.  Kernels that reduce across workgroups finish in the
+// last-arriving workgroup (ds_reduce.hpp), which alone may change the status.
+#include "ds_dev.hpp"
+#include "ds_kernels.hpp"
+#include "ds_reduce.hpp"
+
+namespace pds {
+
+constexpr int BS = 256;
+constexpr int CHUNK = 32;  // eta rows staged in LDS per step of the PFI chains
+
+__device__ inline bool stopped(const DevScalars* S) { return S->status != ST_RUNNING || S->refac_mid != 0; }
+
+// checkVarIsInfeas with its case fallthrough (baseDualSimplex.tpp:443-466).
+__device__ inline bool var_infeas(int8_t t, double x, double l, double u, double& delta) {
+  switch (t) {
+    case BT_LOWER:
+      if (x < l - EPS_BOUND) { delta = x - l; return true; }
+      [[fallthrough]];
+    case BT_UPPER:
+      if (x > u + EPS_BOUND) { delta = x - u; return true; }
+      [[fallthrough]];
+    case BT_FIXED:
+      if (fabs(x - l) < EPS_BOUND) { delta = x - l; return true; }
+      [[fallthrough]];
+    case BT_BOXED:
+      if (x > u + EPS_BOUND) { delta = x - u; return true; }
+      else if (x < l - EPS_BOUND) { delta = x - l; return true; }
+      break;
+    default:
+      break;
+  }
+  delta = 0.0;
+  return false;
+}
+
+// Negation of checkPrimalFeasible for one basic variable (tpp:296-324).
+__device__ inline bool primal_infeas(int8_t t, double x, double l, double u) {
+  switch (t) {
+    case BT_FIXED:
+      if (fabs(x - l) > EPS_BOUND) return true;
+      break;
+    case BT_FREE:
+      break;
+    case BT_BOXED:
+      if (x > u + EPS_R * fabs(u) + EPS_BOUND || x < l - EPS_R * fabs(l) - EPS_BOUND) return true;
+      [[fallthrough]];
+    case BT_UPPER:
+      if (x > u + EPS_R * fabs(u) + EPS_BOUND) return true;
+      [[fallthrough]];
+    case BT_LOWER:
+      if (x < l - EPS_R * fabs(l) - EPS_BOUND) return true;
+      break;
+    default:
+      break;
+  }
+  return false;
+}
+
+// counterDualInfeasible's per-variable test (tpp:415-440).
+__device__ inline int dual_infeas(int8_t t, double d) {
+  switch (t) {
+    case BT_FREE: return (fabs(d) < EPS_D || isnan(d)) ? 1 : 0;
+    case BT_UPPER: return (d > EPS_D || isnan(d)) ? 1 : 0;
+    case BT_LOWER: return (d < -EPS_D || isnan(d)) ? 1 : 0;
+    default: return 0;
+  }
+}
+
+__device__ inline void raise_status(DevScalars* S, int st, int it) {
+  S->status = st;
+  S->stop_iter = it;
+}
+
+// ---------------------------------------------------------------------------
+// CHUZR: DSE pricing (tpp:470-501) in phase 2, phase1Pricing (tpp:746-772) in
+// phase 1, fused with the primal feasibility test of callDualSolver
+// (sequentialDualSimplex.cpp:305-310) and the phase-1 termination tests
+// (tpp:841-875).  Argmax of delta^2/beta, ties to the lowest position.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_price(DevModel M, DevIter I, int phase, int it, int skip_top,
+                                              int perturb_on) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  if (!skip_top) {
+    int st = ST_RUNNING;
+    if (perturb_on && !S->perturbed && S->cycle > MAX_CYCLE) st = ST_NEED_PERTURB;
+    else if (phase == 1 && (S->cycle > NEED_RESTART || S->obj > BIG || isnan(S->obj))) st = ST_PH1_REINIT;
+    if (st != ST_RUNNING) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) raise_status(S, st, it);
+      return;
+    }
+  }
+  double bw = -1.0;
+  int bi = INT_MAX;
+  int pinf = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    int j = I.basis[i];
+    int8_t t = M.bt[j];
+    if (phase == 2) {
+      double x = I.x[j], l = M.l[j], u = M.u[j];
+      pinf |= primal_infeas(t, x, l, u) ? 1 : 0;
+      double dl;
+      if (var_infeas(t, x, l, u, dl) && fabs(dl) > DSE_FLOOR && !I.blocked[j]) {
+        double w = (dl * dl) / I.beta[i];
+        if (w > 0.0) argmax_merge(bw, bi, w, i);
+      }
+    } else {
+      double fi = I.f[i];
+      if ((t == BT_LOWER && fi > EPS_F) || (t == BT_UPPER && fi < -EPS_F) || t == BT_BOXED || t == BT_FIXED) {
+        double w = (fi * fi) / I.beta[i];
+        if (w > 0.0 && !I.blocked[j]) argmax_merge(bw, bi, w, i);
+      }
+    }
+  }
+  block_argmax<BS>(bw, bi);
+  pinf = block_sumi<BS>(pinf);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = bw;
+    I.part_i[2 * blockIdx.x] = bi;
+    I.part_i[2 * blockIdx.x + 1] = pinf;
+  }
+  if (!last_block(&S->ctr[CTR_PRICE])) return;
+  double fw = -1.0;
+  int fi = INT_MAX, fp = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    argmax_merge(fw, fi, I.part_d[b], I.part_i[2 * b]);
+    fp += I.part_i[2 * b + 1];
+  }
+  block_argmax<BS>(fw, fi);
+  fp = block_sumi<BS>(fp);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_PRICE] = 0;
+  bool found = fi != INT_MAX;
+  if (phase == 2) {
+    if (!fp) {
+      raise_status(S, ST_OPTIMAL, it);
+    } else if (!found) {
+      raise_status(S, ST_NO_PIVOT, it);
+    } else {
+      int j = I.basis[fi];
+      double dl;
+      var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl);
+      S->p_idx = fi;
+      S->p = j;
+      S->delta = dl;
+      S->is_lower = dl < 0 ? 1 : 0;
+      S->neg = S->is_lower;
+    }
+  } else {
+    int dinf = S->dinf;
+    double obj = S->obj;
+    int st = ST_RUNNING;
+    if (!found || dinf == 0) {
+      if (fabs(obj) < EPS_A || dinf == 0) st = ST_PH1_SOLVED;
+      else if (obj > 0) st = ST_PH1_DUALINF;
+    }
+    if (st == ST_RUNNING && (!found || obj > EPS_BOUND)) st = ST_PH1_RESTART;
+    if (st != ST_RUNNING) {
+      raise_status(S, st, it);
+    } else {
+      S->p_idx = fi;
+      S->p = I.basis[fi];
+      S->f_p = I.f[fi];
+      S->neg = I.f[fi] > 0 ? 1 : 0;
+    }
+  }
+}
+
+// Periodic dual feasibility check of phase 2 (sequentialDualSimplex.cpp:285-293).
+__global__ __launch_bounds__(BS) void k_dualcheck(DevModel M, DevIter I, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  int cnt = 0;
+  int nN = M.N - M.m;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    int j = I.nonbasis[i];
+    cnt += dual_infeas(M.bt[j], I.d[j]);
+  }
+  cnt = block_sumi<BS>(cnt);
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = cnt;
+  if (!last_block(&S->ctr[CTR_DCHK])) return;
+  int tot = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) tot += I.part_i[b];
+  tot = block_sumi<BS>(tot);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_DCHK] = 0;
+    S->dinf = tot;
+    if (tot > 0) raise_status(S, ST_DUAL_RESTART, it);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BTRAN for a unit vector: rho = B0^{-T} E_1^T ... E_t^T e_p
+// (sequentialDualSimplex.cpp:89-92, linalg.cpp:34-44).  The transposed etas
+// touch only the pivot positions, so the chain runs on a (t+1)-entry sparse
+// vector in one workgroup; the kernel-inverse gather then runs on all rows.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_btran_chain(DevModel M, DevIter I, DevFactor F, int t, int p_fixed) {
+  DevScalars* S = I.S;
+  if (p_fixed < 0 && stopped(S)) return;
+  const int p = p_fixed >= 0 ? p_fixed : S->p_idx;
+  const int m = M.m;
+  __shared__ int pos[ETA_MAX + 1];
+  __shared__ int slot[ETA_MAX + 1];
+  __shared__ double v[ETA_MAX + 1];
+  __shared__ double Ech[CHUNK][ETA_MAX + 1];
+  __shared__ int cnt[ETA_MAX + 1];
+  __shared__ int off[ETA_MAX + 2];
+  const int ns = t + 1;
+  for (int s = threadIdx.x; s < ns; s += BS) pos[s] = (s == 0) ? p : I.eta_pos[s - 1];
+  __syncthreads();
+  for (int s = threadIdx.x; s < ns; s += BS) {
+    int f = s;
+    for (int s2 = 0; s2 < s; ++s2)
+      if (pos[s2] == pos[s]) { f = s2; break; }
+    slot[s] = f;
+    v[s] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) v[slot[0]] = 1.0;
+  __syncthreads();
+  // Wave 0 keeps v in registers: lane l owns slots l, l+64, ...
+  constexpr int RPL = (ETA_MAX + 1 + WAVE - 1) / WAVE;
+  double vr[RPL];
+  const int lane = threadIdx.x % WAVE;
+  const bool w0 = threadIdx.x < WAVE;
+  if (w0)
+    for (int r = 0; r < RPL; ++r) {
+      int s = lane + WAVE * r;
+      vr[r] = (s < ns) ? v[s] : 0.0;
+    }
+  for (int khi = t - 1; khi >= 0; khi -= CHUNK) {
+    int klo = khi - CHUNK + 1;
+    if (klo < 0) klo = 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < (khi - klo + 1) * ns; e += BS) {
+      int kk = e / ns, s = e % ns;
+      int k = klo + kk;
+      Ech[kk][s] = (s == 0) ? I.etas[(size_t)k * m + p] : I.E[(size_t)k * I.eta_cap + (s - 1)];
+    }
+    __syncthreads();
+    if (w0) {
+      for (int k = khi; k >= klo; --k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+          int s = lane + WAVE * r;
+          if (s < ns && slot[s] == s) acc += Ech[k - klo][s] * vr[r];
+        }
+        acc = wave_sum(acc);
+        const int tgt = slot[1 + k];
+#pragma unroll
+        for (int r = 0; r < RPL; ++r)
+          if (tgt == lane + WAVE * r) vr[r] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  if (w0)
+    for (int r = 0; r < RPL; ++r) {
+      int s = lane + WAVE * r;
+      if (s < ns) v[s] = vr[r];
+    }
+  __syncthreads();
+  // Right-hand side of the kernel solve: rhs_P = v_P - C^T y_S, with
+  // y_S = v at slack positions (ordered by slot, prefix-summed offsets).
+  for (int s = threadIdx.x; s < ns; s += BS) {
+    int c = 0;
+    if (slot[s] == s && v[s] != 0.0) {
+      int ps = pos[s];
+      if (F.Pidx_of_pos[ps] >= 0) c = 1;
+      else {
+        int r = F.slackrow_of_pos[ps];
+        c = 1 + (F.crp[r + 1] - F.crp[r]);
+      }
+    }
+    cnt[s] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0;
+    for (int s = 0; s < ns; ++s) { off[s] = a; a += cnt[s]; }
+    off[ns] = a;
+  }
+  __syncthreads();
+  const int total = off[ns];
+  if (total > I.list_cap) {
+    if (threadIdx.x == 0) raise_status(S, ST_BTRAN_CAP, -1);
+    return;
+  }
+  for (int s = threadIdx.x; s < ns; s += BS) {
+    if (cnt[s] == 0) continue;
+    int ps = pos[s];
+    double vs = v[s];
+    int o = off[s];
+    int c = F.Pidx_of_pos[ps];
+    if (c >= 0) {
+      I.list_idx[o] = c;
+      I.list_val[o] = vs;
+    } else {
+      int r = F.slackrow_of_pos[ps];
+      I.list_idx[o] = -1 - r;   // marks a slack row entry (no kernel contribution)
+      I.list_val[o] = vs;
+      for (int e = F.crp[r]; e < F.crp[r + 1]; ++e) {
+        ++o;
+        I.list_idx[o] = F.cci[e];
+        I.list_val[o] = -F.ccv[e] * vs;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0;
+    for (int s = 0; s < ns; ++s) {
+      if (cnt[s] == 0) continue;
+      int c = F.Pidx_of_pos[pos[s]];
+      if (c < 0) {
+        I.ylist_row[a] = F.slackrow_of_pos[pos[s]];
+        I.ylist_val[a] = v[s];
+        ++a;
+      }
+    }
+    I.list_n[0] = total;
+    I.list_n[1] = a;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFactor F, double* out, int check) {
+  DevScalars* S = I.S;
+  if (check && stopped(S)) return;
+  __shared__ int sidx[1024];
+  __shared__ double sval[1024];
+  __shared__ int yrow[ETA_MAX + 1];
+  __shared__ double yval[ETA_MAX + 1];
+  const int total = I.list_n[0];
+  const int ny = I.list_n[1];
+  for (int s = threadIdx.x; s < ny; s += BS) {
+    yrow[s] = I.ylist_row[s];
+    yval[s] = I.ylist_val[s];
+  }
+  const int r = blockIdx.x * BS + threadIdx.x;
+  const bool live = r < M.m;
+  const int ri = live ? F.Ridx_of_row[r] : -1;
+  double acc = 0.0;
+  for (int base = 0; base < total; base += 1024) {
+    int nchunk = min(1024, total - base);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nchunk; e += BS) {
+      sidx[e] = I.list_idx[base + e];
+      sval[e] = I.list_val[base + e];
+    }
+    __syncthreads();
+    if (live && ri >= 0)
+      for (int e = 0; e < nchunk; ++e) {
+        int c = sidx[e];
+        if (c >= 0) acc += sval[e] * F.Kinv[(size_t)c * F.k + ri];
+      }
+  }
+  __syncthreads();
+  if (!live) return;
+  if (ri < 0) {
+    acc = 0.0;
+    for (int s = 0; s < ny; ++s)
+      if (yrow[s] == r) acc = yval[s];
+  }
+  out[r] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Pivot-row SpMV: Matrix::dotUpdate UPDATE_T (matrix.cpp:273-287) over the
+// CSC of [A | I] (= CSR of A^T).  One wave per 64 consecutive columns: the
+// wave streams the columns' contiguous (row, value) segment, forms the
+// products into LDS, and each lane then sums its own column in CSC order, so
+// the result is bitwise that of the sequential loop.  Basic columns are
+// skipped at the store.  out index = nonbasic position, or the column id when
+// by_col (the `set` flag).
+// ---------------------------------------------------------------------------
+constexpr int SPMV_SEG = 1024;
+__global__ __launch_bounds__(WAVE) void k_spmv(DevModel M, const int* __restrict__ posN, const double* __restrict__ v,
+                                               double alpha, double beta, const double* __restrict__ cvec,
+                                               double* __restrict__ out, int by_col, const DevScalars* S, int check) {
+  if (check && stopped(S)) return;
+  __shared__ double prod[SPMV_SEG];
+  const int lane = threadIdx.x;
+  const int j0 = blockIdx.x * WAVE;
+  const int jend = min(j0 + WAVE, M.N);
+  const int j = j0 + lane;
+  const int e0 = M.cptr[j0];
+  const int e1 = M.cptr[jend];
+  int pn = -1, cb = 0, ce = 0;
+  if (j < jend) {
+    pn = posN[j];
+    cb = M.cptr[j];
+    ce = M.cptr[j + 1];
+  }
+  if (e1 - e0 <= SPMV_SEG) {
+    for (int e = e0 + lane; e < e1; e += WAVE) prod[e - e0] = M.cval[e] * v[M.ridx[e]] * alpha;
+    __syncthreads();
+    if (j < jend && (pn >= 0)) {
+      double acc = 0.0;
+      for (int e = cb; e < ce; ++e) acc = acc + prod[e - e0];
+      out[by_col ? j : pn] = acc + cvec[j] * beta;
+    }
+  } else if (j < jend && pn >= 0) {
+    double acc = 0.0;
+    for (int e = cb; e < ce; ++e) acc = acc + M.cval[e] * v[M.ridx[e]] * alpha;
+    out[by_col ? j : pn] = acc + cvec[j] * beta;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CHUZC.  Pass 1 bounds the step: textbook = min d_j/alpha_j over the
+// candidate set (sequentialDualSimplex.cpp:342-367, tpp:531-542, tpp:546-572);
+// Harris = min of the EPS_D-relaxed ratios.  Pass 2 picks, among candidates
+// within the bound, the largest |alpha_j| (lowest position on ties).
+// ---------------------------------------------------------------------------
+__device__ inline bool ratio_candidate(const DevModel& M, const DevIter& I, int phase, int j, double a) {
+  if (phase == 2) {
+    int8_t t = M.bt[j];
+    if (a > EPS_A) return fabs(I.x[j] - M.l[j]) < EPS_Z && (t == BT_LOWER || t == BT_BOXED);
+    if (a < -EPS_A) return fabs(I.x[j] - M.u[j]) < EPS_Z && (t == BT_UPPER || t == BT_BOXED);
+    return t == BT_FREE;
+  }
+  double dj = I.d[j];
+  return (dj >= 0 && a > EPS_A) || (dj <= 0 && a < -EPS_A);
+}
+
+__global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase, int harris, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int neg = S->neg;
+  const int nN = M.N - M.m;
+  double mn = BIG;
+  int nF = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    double a = I.alpha_r[i];
+    if (neg) a = -a;
+    int j = I.nonbasis[i];
+    if (!ratio_candidate(M, I, phase, j, a)) continue;
+    ++nF;
+    double dj = I.d[j];
+    double th = harris ? (a > 0 ? (dj + EPS_D) / a : (dj - EPS_D) / a) : dj / a;
+    mn = fmin(mn, th);
+  }
+  mn = block_min<BS>(mn);
+  nF = block_sumi<BS>(nF);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = mn;
+    I.part_i[blockIdx.x] = nF;
+  }
+  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  double g = BIG;
+  int tot = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    g = fmin(g, I.part_d[b]);
+    tot += I.part_i[b];
+  }
+  g = block_min<BS>(g);
+  tot = block_sumi<BS>(tot);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_CHUZC1] = 0;
+    S->nF = tot;
+    S->tmp0 = g;
+    if (tot == 0) raise_status(S, ST_NO_RATIO, it);
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_chuzc2(DevModel M, DevIter I, int phase, int harris, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int neg = S->neg;
+  const int nN = M.N - M.m;
+  const double bound = S->tmp0;
+  double best = -1.0;
+  int bi = INT_MAX;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    double a = I.alpha_r[i];
+    if (neg) a = -a;
+    int j = I.nonbasis[i];
+    if (!ratio_candidate(M, I, phase, j, a)) continue;
+    double th = I.d[j] / a;
+    bool ok = harris ? (th <= bound) : (fabs(th - bound) < EPS_Z);
+    if (ok) argmax_merge(best, bi, fabs(a), i);
+  }
+  block_argmax<BS>(best, bi);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = best;
+    I.part_i[blockIdx.x] = bi;
+  }
+  if (!last_block(&S->ctr[CTR_CHUZC2])) return;
+  double fb = -1.0;
+  int fi = INT_MAX;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fb, fi, I.part_d[b], I.part_i[b]);
+  block_argmax<BS>(fb, fi);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_CHUZC2] = 0;
+  if (fi == INT_MAX) {
+    raise_status(S, ST_NO_RATIO, it);
+    return;
+  }
+  const int q = I.nonbasis[fi];
+  const double arq = I.alpha_r[fi];
+  S->q_idx = fi;
+  S->q = q;
+  S->alpha_rq = arq;
+  S->theta = I.d[q] / arq;
+  if (phase == 2) {
+    S->unbound_checked = 0;
+    if (M.bt[q] == BT_BOXED) {
+      double span = M.u[q] - M.l[q];
+      if (fabs(S->delta) - span * fabs(arq) > 0) raise_status(S, ST_BFRT, it);
+    }
+  } else {
+    int8_t tq = M.bt[q];
+    double dq = I.d[q];
+    double corr = 0.0;
+    if ((tq == BT_UPPER || tq == BT_FREE) && dq > EPS_D) corr = 1.0;
+    else if ((tq == BT_LOWER || tq == BT_FREE) && dq < -EPS_D) corr = -1.0;
+    S->corr = corr;
+    if (S->any_blocked) {
+      for (int b = 0; b < S->nblocked; ++b) I.blocked[I.blocked_list[b]] = 0;
+      S->nblocked = 0;
+      S->any_blocked = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FTRAN of a sparse column: x0 = B0^{-1} a_j.  Kernel rows of the column
+// select rows of K^{-T}; slack positions subtract the coupling block.
+// ---------------------------------------------------------------------------
+constexpr int FT_MAXCOL = 512;
+__global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFactor F, int col, double* out,
+                                                  int check) {
+  DevScalars* S = I.S;
+  if (check && stopped(S)) return;
+  const int j = col >= 0 ? col : S->q;
+  __shared__ int er[FT_MAXCOL];
+  __shared__ int eri[FT_MAXCOL];
+  __shared__ double ev[FT_MAXCOL];
+  const int cb = M.cptr[j];
+  const int ne = min(M.cptr[j + 1] - cb, FT_MAXCOL);
+  for (int e = threadIdx.x; e < ne; e += BS) {
+    int r = M.ridx[cb + e];
+    er[e] = r;
+    eri[e] = F.Ridx_of_row[r];
+    ev[e] = M.cval[cb + e];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= M.m) return;
+  const int k = F.k;
+  int c = F.Pidx_of_pos[i];
+  double x;
+  if (c >= 0) {
+    x = 0.0;
+    for (int e = 0; e < ne; ++e)
+      if (eri[e] >= 0) x += ev[e] * F.KinvT[(size_t)eri[e] * k + c];
+  } else {
+    int r = F.slackrow_of_pos[i];
+    double ar = 0.0;
+    for (int e = 0; e < ne; ++e)
+      if (er[e] == r) ar = ev[e];
+    double s = 0.0;
+    for (int q = F.crp[r]; q < F.crp[r + 1]; ++q) {
+      int c2 = F.cci[q];
+      double xp = 0.0;
+      for (int e = 0; e < ne; ++e)
+        if (eri[e] >= 0) xp += ev[e] * F.KinvT[(size_t)eri[e] * k + c2];
+      s += F.ccv[q] * xp;
+    }
+    x = ar - s;
+  }
+  out[i] = x;
+}
+
+// Forward PFI chain: multipliers s_k = value at eta_pos[k] just before eta k
+// is applied (linalg.cpp:21-32), from x0 at the pivot positions and the
+// eta values there (E).  One workgroup; the update of all positions follows
+// in k_ftran_out.
+__global__ __launch_bounds__(BS) void k_ftran_chain(DevModel M, DevIter I, const double* x0, int t, int check) {
+  DevScalars* S = I.S;
+  if (check && stopped(S)) return;
+  __shared__ int pk[ETA_MAX];
+  __shared__ int last[ETA_MAX];
+  __shared__ double xp[ETA_MAX];
+  __shared__ double Ec[ETA_MAX][CHUNK + 1];
+  __shared__ double s_sh[ETA_MAX];
+  for (int k = threadIdx.x; k < t; k += BS) {
+    pk[k] = I.eta_pos[k];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS) {
+    int lst = -1;
+    for (int k2 = k - 1; k2 >= 0; --k2)
+      if (pk[k2] == pk[k]) { lst = k2; break; }
+    last[k] = lst;
+    xp[k] = x0[pk[k]];
+  }
+  const int lane = threadIdx.x % WAVE;
+  const bool w0 = threadIdx.x < WAVE;
+  for (int k0 = 0; k0 < t; k0 += CHUNK) {
+    int k1 = min(t, k0 + CHUNK);
+    __syncthreads();
+    // Ec[j][kk] = E[j][k0+kk] = eta_j[p_{k0+kk}] for j < k1
+    for (int e = threadIdx.x; e < k1 * (k1 - k0); e += BS) {
+      int jj = e / (k1 - k0), kk = e % (k1 - k0);
+      Ec[jj][kk] = I.E[(size_t)jj * I.eta_cap + (k0 + kk)];
+    }
+    __syncthreads();
+    if (w0) {
+      for (int k = k0; k < k1; ++k) {
+        int lst = last[k];
+        double acc = 0.0;
+        for (int jj = lst + 1 + lane; jj < k; jj += WAVE) acc += s_sh[jj] * Ec[jj][k - k0];
+        acc = wave_sum(acc);
+        double base = (lst < 0) ? xp[k] : s_sh[lst] * Ec[lst][k - k0];
+        if (lane == 0) s_sh[k] = base + acc;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS) I.chain[k] = s_sh[k];
+}
+
+// x = x0 with the forward etas applied, using the chain multipliers.
+// mode 0: out[i] = x; mode 1: x_var[basis[i]] = x (dualSimplexInit's scatter).
+__global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* x0, double* out, int t,
+                                                  int mode, int check) {
+  DevScalars* S = I.S;
+  if (check && stopped(S)) return;
+  __shared__ double sk[ETA_MAX];
+  __shared__ int pk[ETA_MAX];
+  for (int k = threadIdx.x; k < t; k += BS) {
+    sk[k] = I.chain[k];
+    pk[k] = I.eta_pos[k];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= M.m) return;
+  double x = x0[i];
+  for (int k = 0; k < t; ++k) {
+    double s = sk[k];
+    if (pk[k] == i) x = s * I.etas[(size_t)k * M.m + i];
+    else if (s != 0.0) x = x + s * I.etas[(size_t)k * M.m + i];
+  }
+  if (mode == 0) out[i] = x;
+  else out[I.basis[i]] = x;
+}
+
+// Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
+__global__ __launch_bounds__(BS) void k_gather(const double* src, const int* map, double* dst, int len,
+                                               const DevScalars* S, int check) {
+  if (check && stopped(S)) return;
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i < len) dst[i] = src[map[i]];
+}
+
+__global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int rows, int cols,
+                                             const double* __restrict__ xin, double* __restrict__ dst,
+                                             const int* omap, const DevScalars* S, int check) {
+  if (check && stopped(S)) return;
+  const int w = (blockIdx.x * BS + threadIdx.x) / WAVE;
+  const int lane = threadIdx.x % WAVE;
+  if (w >= rows) return;
+  const double* row = A + (size_t)w * cols;
+  double acc = 0.0;
+  for (int jj = lane; jj < cols; jj += WAVE) acc += row[jj] * xin[jj];
+  acc = wave_sum(acc);
+  if (lane == 0) dst[omap ? omap[w] : w] = acc;
+}
+
+__global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, const double* b, double* dst,
+                                                    const DevScalars* S, int check) {
+  if (check && stopped(S)) return;
+  int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= M.m) return;
+  int sp = F.slackpos_of_row[r];
+  if (sp < 0) return;
+  double s = 0.0;
+  for (int q = F.crp[r]; q < F.crp[r + 1]; ++q) s += F.ccv[q] * dst[F.pos_P[F.cci[q]]];
+  dst[sp] = b[r] - s;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-cost update (sequentialDualSimplex.cpp:428-467, and
+// simpleReducedCostsUpate :163-172 in phase 1) with the stability check.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double theta = S->theta;
+  const int q_idx = S->q_idx;
+  const int nN = M.N - M.m;
+  int dinf = 0, flips = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    int j = I.nonbasis[i];
+    double dj = I.d[j] - theta * I.alpha_r[i];
+    I.d[j] = dj;
+    if (i != q_idx) dinf += dual_infeas(M.bt[j], dj);
+    if (phase == 2 && M.bt[j] == BT_BOXED) {
+      double xj = I.x[j];
+      if ((fabs(xj - M.l[j]) < EPS_Z && dj < -EPS_D) || (fabs(xj - M.u[j]) < EPS_Z && dj > EPS_D)) ++flips;
+    }
+  }
+  dinf = block_sumi<BS>(dinf);
+  flips = block_sumi<BS>(flips);
+  if (threadIdx.x == 0) {
+    I.part_i[2 * blockIdx.x] = dinf;
+    I.part_i[2 * blockIdx.x + 1] = flips;
+  }
+  if (!last_block(&S->ctr[CTR_DUPD])) return;
+  int td = 0, tf = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    td += I.part_i[2 * b];
+    tf += I.part_i[2 * b + 1];
+  }
+  td = block_sumi<BS>(td);
+  tf = block_sumi<BS>(tf);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_DUPD] = 0;
+  const int p = S->p, q = S->q;
+  I.d[p] = -theta;
+  I.d[q] = 0.0;
+  S->dinf = td + dual_infeas(M.bt[p], -theta);
+  const double aqp = I.alpha_q[S->p_idx];
+  if (fabs(aqp - S->alpha_rq) > EPS_R * (1 + fabs(aqp))) {
+    S->refac_mid = 1;
+    S->stop_iter = it;
+  }
+  if (phase == 2) {
+    if (tf > 0) raise_status(S, ST_BFRT, it);
+    S->obj = S->obj + theta * S->delta;
+  } else {
+    S->obj = S->obj - theta * S->f_p;
+  }
+  S->alpha_p = aqp;
+}
+
+// ---------------------------------------------------------------------------
+// Basis change: x_B (or f) update, new eta, DSE weights, swap
+// (sequentialDualSimplex.cpp:174-227).  The finaliser appends the eta's
+// pivot and its row/column of the E table.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_update(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int p_idx = S->p_idx;
+  const double ap = I.alpha_q[p_idx];
+  const double bp = I.beta[p_idx];
+  const double thP = (phase == 2 ? S->delta : S->f_p) / ap;
+  const int t = S->t;
+  double* eta = I.etas + (size_t)t * M.m;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    double ai = I.alpha_q[i];
+    if (phase == 2) {
+      int j = I.basis[i];
+      I.x[j] = I.x[j] - thP * ai;
+    } else if (i != p_idx) {
+      I.f[i] = I.f[i] - ai * thP;
+    }
+    if (i != p_idx) {
+      eta[i] = -ai / ap;
+      double r = ai / ap;
+      I.beta[i] = I.beta[i] - 2 * ai / ap * I.tau[i] + (r * r) * bp;
+    } else {
+      eta[i] = 1 / ap;
+    }
+  }
+  if (!last_block(&S->ctr[CTR_UPD])) return;
+  // New E entries: row t (eta_t at the older pivots) and column t.
+  const int cap = I.eta_cap;
+  for (int jj = threadIdx.x; jj < t; jj += BS) {
+    I.E[(size_t)t * cap + jj] = eta[I.eta_pos[jj]];
+    I.E[(size_t)jj * cap + t] = I.etas[(size_t)jj * M.m + p_idx];
+  }
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_UPD] = 0;
+  I.E[(size_t)t * cap + t] = eta[p_idx];
+  I.eta_pos[t] = p_idx;
+  S->t = t + 1;
+  I.beta[p_idx] = bp / (ap * ap);
+  const int p = S->p, q = S->q, q_idx = S->q_idx;
+  if (phase == 2) {
+    I.x[q] = I.x[q] + thP;
+  } else {
+    I.f[p_idx] = I.f[p_idx] / ap + S->corr;
+  }
+  I.basis[p_idx] = q;
+  I.nonbasis[q_idx] = p;
+  I.posN[q] = -1;
+  I.posN[p] = q_idx;
+  S->theta_P = thP;
+  const double theta = S->theta;
+  if (phase == 2) {
+    if (fabs(theta) < EPS_A) S->cycle = S->cycle + 1;
+  } else {
+    S->cycle = (fabs(theta) < EPS_A) ? S->cycle + 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation helpers.
+// ---------------------------------------------------------------------------
+
+// setPrimalVars (tpp:74-101) over the nonbasic set.
+__global__ __launch_bounds__(BS) void k_set_primal(DevModel M, DevIter I) {
+  int nN = M.N - M.m;
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= nN) return;
+  int j = I.nonbasis[i];
+  double v = 0.0;
+  switch (M.bt[j]) {
+    case BT_FIXED: v = M.l[j]; break;
+    case BT_FREE: v = 0.0; break;
+    case BT_BOXED: v = (I.d[j] > 0) ? M.l[j] : M.u[j]; break;
+    case BT_UPPER: v = M.u[j]; break;
+    case BT_LOWER: v = M.l[j]; break;
+  }
+  I.x[j] = v;
+}
+
+// wN[j] = x[j] for nonbasic j, else 0 (Matrix::dotUpdate UPDATE's buffer).
+__global__ __launch_bounds__(BS) void k_mask_nonbasic(DevModel M, DevIter I, const double* src, double* wN) {
+  int j = blockIdx.x * BS + threadIdx.x;
+  if (j >= M.N) return;
+  wN[j] = (I.posN[j] >= 0) ? src[j] : 0.0;
+}
+
+// Row-wise SpMV (Matrix::dotUpdate UPDATE, matrix.cpp:330-341):
+// out[r] = b[r]*beta + sum_e val_e * w[col_e] * alpha in CSR order.
+__global__ __launch_bounds__(BS) void k_rowspmv(DevModel M, const double* w, const double* b, double alpha,
+                                                double beta, double* out) {
+  int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= M.m) return;
+  double s = (b ? b[r] : 0.0) * beta;
+  for (int e = M.rptr[r]; e < M.rptr[r + 1]; ++e) s = s + M.rval[e] * w[M.cidx[e]] * alpha;
+  out[r] = s;
+}
+
+// obj = c . x over all variables.
+__global__ __launch_bounds__(BS) void k_dot_obj(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  double s = 0.0;
+  for (int j = blockIdx.x * BS + threadIdx.x; j < M.N; j += gridDim.x * BS) s += M.c[j] * I.x[j];
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) I.part_d[blockIdx.x] = s;
+  if (!last_block(&S->ctr[CTR_DOT])) return;
+  double g = 0.0;
+  if (threadIdx.x == 0)
+    for (int b = 0; b < (int)gridDim.x; ++b) g += I.part_d[b];
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_DOT] = 0;
+    S->obj = g;
+  }
+}
+
+// Phase-1 infeasibility sets (tpp:776-795, sequentialDualSimplex.cpp:135-152):
+// wN[j] = +1 for inf_l, -1 for inf_u, else 0; Z = sum w_j d_j; dinf count.
+__global__ __launch_bounds__(BS) void k_phase1_inf(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  double z = 0.0;
+  int cnt = 0;
+  for (int j = blockIdx.x * BS + threadIdx.x; j < M.N; j += gridDim.x * BS) {
+    double w = 0.0;
+    if (I.posN[j] >= 0) {
+      int8_t t = M.bt[j];
+      double dj = I.d[j];
+      if ((t == BT_UPPER || t == BT_FREE) && dj > EPS_D) w = -1.0;
+      else if ((t == BT_LOWER || t == BT_FREE) && dj < -EPS_D) w = 1.0;
+      z += w * dj;
+      cnt += dual_infeas(t, dj);
+    }
+    I.wN[j] = w;
+  }
+  z = block_sum<BS>(z);
+  cnt = block_sumi<BS>(cnt);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = z;
+    I.part_i[blockIdx.x] = cnt;
+  }
+  if (!last_block(&S->ctr[CTR_INF])) return;
+  if (threadIdx.x == 0) {
+    double g = 0.0;
+    int c = 0;
+    for (int b = 0; b < (int)gridDim.x; ++b) {
+      g += I.part_d[b];
+      c += I.part_i[b];
+    }
+    S->ctr[CTR_INF] = 0;
+    S->obj = g;
+    S->dinf = c;
+  }
+}
+
+// w1[i] = c[basis[i]]
+__global__ __launch_bounds__(BS) void k_gather_cb(DevModel M, DevIter I, double* out) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i < M.m) out[i] = M.c[I.basis[i]];
+}
+
+// Transposed etas on a dense vector (linalg.cpp:34-44), one workgroup.  The
+// modified pivot entries live in LDS so no thread reads a value another
+// thread of the workgroup wrote to global memory.
+__global__ __launch_bounds__(1024) void k_btran_dense_eta(DevModel M, DevIter I, double* v, int t) {
+  __shared__ int mpos[ETA_MAX];
+  __shared__ double mval[ETA_MAX];
+  __shared__ double morig[ETA_MAX];
+  __shared__ int nmod;
+  __shared__ double red[16];
+  if (threadIdx.x == 0) nmod = 0;
+  __syncthreads();
+  for (int k = t - 1; k >= 0; --k) {
+    const double* eta = I.etas + (size_t)k * M.m;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < M.m; i += 1024) s += eta[i] * v[i];
+    s = wave_sum(s);
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double dot = 0.0;
+      for (int w = 0; w < 16; ++w) dot += red[w];
+      for (int q = 0; q < nmod; ++q) dot += eta[mpos[q]] * (mval[q] - morig[q]);
+      int pk = I.eta_pos[k];
+      int f = -1;
+      for (int q = 0; q < nmod; ++q)
+        if (mpos[q] == pk) f = q;
+      if (f < 0) {
+        f = nmod++;
+        mpos[f] = pk;
+        morig[f] = v[pk];
+      }
+      mval[f] = dot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nmod; ++q) v[mpos[q]] = mval[q];
+}
+
+// rhs_P[c] = v[pos_P[c]] - sum_{(r,val) in C column c} val * v[slackpos(r)]
+__global__ __launch_bounds__(BS) void k_btran_dense_rhs(DevFactor F, const double* v, double* rhs) {
+  int c = blockIdx.x * BS + threadIdx.x;
+  if (c >= F.k) return;
+  double s = v[F.pos_P[c]];
+  for (int q = F.ccp[c]; q < F.ccp[c + 1]; ++q) s = s - F.ccv2[q] * v[F.slackpos_of_row[F.ccr[q]]];
+  rhs[c] = s;
+}
+
+// y[r] = v[slackpos(r)] for covered rows (kernel rows come from the GEMV).
+__global__ __launch_bounds__(BS) void k_btran_dense_slack(DevModel M, DevFactor F, const double* v, double* y) {
+  int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= M.m) return;
+  int sp = F.slackpos_of_row[r];
+  if (sp >= 0) y[r] = v[sp];
+}
+
+// Exact DSE weights right after a refactor (initBetaWeights(false),
+// tpp:41-68): beta_i = ||e_i^T B0^{-1}||^2.  One workgroup per position.
+__global__ __launch_bounds__(BS) void k_dse_exact(DevModel M, DevIter I, DevFactor F) {
+  const int i = blockIdx.x;
+  if (i >= M.m) return;
+  const int k = F.k;
+  int c = F.Pidx_of_pos[i];
+  double s = 0.0;
+  if (c >= 0) {
+    const double* row = F.Kinv + (size_t)c * k;
+    for (int jj = threadIdx.x; jj < k; jj += BS) s += row[jj] * row[jj];
+  } else {
+    int r = F.slackrow_of_pos[i];
+    for (int jj = threadIdx.x; jj < k; jj += BS) {
+      double g = 0.0;
+      for (int q = F.crp[r]; q < F.crp[r + 1]; ++q) g += F.ccv[q] * F.Kinv[(size_t)F.cci[q] * k + jj];
+      s += g * g;
+    }
+  }
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) I.beta[i] = (c >= 0) ? s : 1.0 + s;
+}
+
+__global__ __launch_bounds__(BS) void k_fill(double* p, double v, int n) {
+  int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+static inline int cap_grid(long n, int bs, int cap) {
+  int g = nblk(n, bs);
+  if (g < 1) g = 1;
+  return g > cap ? cap : g;
+}
+
+void launch_price(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it, int skip_top, int perturb_on) {
+  k_price<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it, skip_top, perturb_on);
+}
+void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
+  k_dualcheck<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
+}
+void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
+                       int p_fixed) {
+  k_btran_chain<<<1, BS, 0, s>>>(M, I, F, t, p_fixed);
+  k_btran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0);
+}
+void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
+                 const double* cvec, double* out, int by_col, const DevScalars* S, int check) {
+  k_spmv<<<nblk(M.N, WAVE), WAVE, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
+}
+void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
+  int g = cap_grid(M.N - M.m, BS, I.part_cap);
+  k_chuzc1<<<g, BS, 0, s>>>(M, I, phase, harris, it);
+  k_chuzc2<<<g, BS, 0, s>>>(M, I, phase, harris, it);
+}
+void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
+                      double* out, int check) {
+  k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, out, check);
+  if (t > 0) {
+    k_ftran_chain<<<1, BS, 0, s>>>(M, I, out, t, check);
+    k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, out, out, t, 0, check);
+  }
+}
+void launch_ftran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const double* b,
+                        double* out, int t, int mode, int check) {
+  // out (by position) = B^{-1} b; mode 1 scatters the result into x via basis.
+  double* tmp = (mode == 0) ? out : I.w2;
+  if (F.k > 0) {
+    k_gather<<<nblk(F.k, BS), BS, 0, s>>>(b, F.row_R, I.wN, F.k, I.S, check);
+    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.Kinv, F.k, F.k, I.wN, tmp, F.pos_P, I.S, check);
+  }
+  k_ftran_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, b, tmp, I.S, check);
+  if (t > 0) k_ftran_chain<<<1, BS, 0, s>>>(M, I, tmp, t, check);
+  if (t > 0 || mode == 1) k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, tmp, out, t, mode, check);
+}
+void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, double* v,
+                        double* y, int t) {
+  // v (by position, overwritten) -> y (by row) = B^{-T} v
+  if (t > 0) k_btran_dense_eta<<<1, 1024, 0, s>>>(M, I, v, t);
+  if (F.k > 0) {
+    k_btran_dense_rhs<<<nblk(F.k, BS), BS, 0, s>>>(F, v, I.wN);
+    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.KinvT, F.k, F.k, I.wN, y, F.row_R, I.S, 0);
+  }
+  k_btran_dense_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, v, y);
+}
+void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
+  k_dupdate<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
+}
+void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
+  k_update<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
+}
+void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I) {
+  k_set_primal<<<nblk(M.N - M.m, BS), BS, 0, s>>>(M, I);
+}
+void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN) {
+  k_mask_nonbasic<<<nblk(M.N, BS), BS, 0, s>>>(M, I, src, wN);
+}
+void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,
+                    double* out) {
+  k_rowspmv<<<nblk(M.m, BS), BS, 0, s>>>(M, w, b, alpha, beta, out);
+}
+void launch_dot_obj(hipStream_t s, const DevModel& M, const DevIter& I) {
+  k_dot_obj<<<cap_grid(M.N, BS, I.part_cap), BS, 0, s>>>(M, I);
+}
+void launch_phase1_inf(hipStream_t s, const DevModel& M, const DevIter& I) {
+  k_phase1_inf<<<cap_grid(M.N, BS, I.part_cap), BS, 0, s>>>(M, I);
+}
+void launch_gather_cb(hipStream_t s, const DevModel& M, const DevIter& I, double* out) {
+  k_gather_cb<<<nblk(M.m, BS), BS, 0, s>>>(M, I, out);
+}
+void launch_dse_exact(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F) {
+  k_dse_exact<<<M.m, BS, 0, s>>>(M, I, F);
+}
+void launch_fill(hipStream_t s, double* p, double v, int n) {
+  if (n > 0) k_fill<<<nblk(n, BS), BS, 0, s>>>(p, v, n);
+}
+
+}  // namespace pds

@@ -1,0 +1,48 @@
+// Host-callable launchers of the gfx950 kernels (asynchronous on `s`).
+#pragma once
+#include "ds_dev.hpp"
+
+namespace pds {
+
+constexpr int ETA_MAX = 256;  // eta file capacity (refactor interval + margin)
+
+void launch_price(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it, int skip_top, int perturb_on);
+void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it);
+void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
+                       int p_fixed);
+void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
+                 const double* cvec, double* out, int by_col, const DevScalars* S, int check);
+void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it);
+void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
+                      double* out, int check);
+void launch_ftran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const double* b,
+                        double* out, int t, int mode, int check);
+void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, double* v,
+                        double* y, int t);
+void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
+void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
+void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I);
+void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN);
+void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,
+                    double* out);
+void launch_dot_obj(hipStream_t s, const DevModel& M, const DevIter& I);
+void launch_phase1_inf(hipStream_t s, const DevModel& M, const DevIter& I);
+void launch_gather_cb(hipStream_t s, const DevModel& M, const DevIter& I, double* out);
+void launch_dse_exact(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F);
+void launch_fill(hipStream_t s, double* p, double v, int n);
+
+// Kernel inverse by blocked Gauss-Jordan (ds_refactor.hip).
+struct GJWork {
+  double* W;      // k x k work matrix (row-major, ld = k)
+  double* Pf;     // k x NB panel LU factors
+  double* Pb;     // k x NB saved panel
+  int* piv;       // k
+  int* perm;      // k
+  int* flag;      // singular-pivot flag
+  int use_mfma;   // trailing update on FP64 MFMA (1) or plain FMA (0)
+};
+constexpr int GJ_NB = 64;
+void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, const DevFactor& F, double* W);
+void launch_gj_invert(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT);
+
+}  // namespace pds

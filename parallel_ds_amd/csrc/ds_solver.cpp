@@ -1,0 +1,914 @@
+// Host driver of the MI355X dual simplex.
+//
+// The control flow is that of the reference's sequential solver
+// (cpu/sequentialDualSimplex.cpp:253-515 for phase 2,
+// common/baseDualSimplex.tpp:799-934 for phase 1); the arithmetic of every
+// iteration runs in gfx950 kernels on device-resident state.  The host queues
+// up to `chunk` iterations at a time without synchronising; a kernel that
+// meets an event (optimality, restart, refactor request, ...) raises the
+// device status word, the remaining queued kernels return at entry, and the
+// host handles the event exactly where the reference would.
+#include "ds_solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "ds_rng.hpp"
+
+namespace pds {
+
+namespace {
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void HostModel::build(int m_, int n_, const int* col_ptr, const int* row_idx, const double* val,
+                      const double* costs, const double* lb, const double* ub, const int* bound_type,
+                      const double* lr, const double* ur, const int* range_type) {
+  m = m_;
+  n = n_;
+  N = n + m;
+  const int nnzA = col_ptr[n];
+  cptr.assign(N + 1, 0);
+  ridx.resize(nnzA + m);
+  cval.resize(nnzA + m);
+  for (int j = 0; j <= n; ++j) cptr[j] = col_ptr[j];
+  for (int e = 0; e < nnzA; ++e) {
+    ridx[e] = row_idx[e];
+    cval[e] = val[e];
+  }
+  for (int i = 0; i < m; ++i) {
+    ridx[nnzA + i] = i;
+    cval[nnzA + i] = 1.0;
+    cptr[n + i + 1] = nnzA + i + 1;
+  }
+  rptr.assign(m + 1, 0);
+  for (int e = 0; e < nnzA + m; ++e) rptr[ridx[e] + 1]++;
+  for (int i = 0; i < m; ++i) rptr[i + 1] += rptr[i];
+  cidx.resize(nnzA + m);
+  rval.resize(nnzA + m);
+  std::vector<int> fill(rptr.begin(), rptr.end() - 1);
+  for (int j = 0; j < N; ++j)
+    for (int e = cptr[j]; e < cptr[j + 1]; ++e) {
+      int r = ridx[e];
+      cidx[fill[r]] = j;
+      rval[fill[r]] = cval[e];
+      fill[r]++;
+    }
+  c.resize(N);
+  l.resize(N);
+  u.resize(N);
+  bt.resize(N);
+  for (int j = 0; j < n; ++j) {
+    c[j] = costs[j];
+    l[j] = lb[j];
+    u[j] = ub[j];
+    bt[j] = (int8_t)bound_type[j];
+  }
+  // Slack of row i: bounds [-upper_range, -lower_range]; a <= row (range
+  // type Upper) gives a Lower slack, >= gives Upper, = Fixed, range Boxed.
+  for (int i = 0; i < m; ++i) {
+    c[n + i] = 0.0;
+    u[n + i] = -lr[i];
+    l[n + i] = -ur[i];
+    int8_t t;
+    switch (range_type[i]) {
+      case DS_FIXED: t = BT_FIXED; break;
+      case DS_BOXED: t = BT_BOXED; break;
+      case DS_LOWER: t = BT_UPPER; break;
+      case DS_UPPER: t = BT_LOWER; break;
+      default: t = BT_FREE; break;
+    }
+    bt[n + i] = t;
+  }
+}
+
+Solver::Solver(int device) : device_(device) {
+  PDS_HIP(hipSetDevice(device_));
+  PDS_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  PDS_HIP(hipHostMalloc((void**)&hS_, sizeof(DevScalars), hipHostMallocDefault));
+  std::memset(hS_, 0, sizeof(DevScalars));
+  ds_default_params(&P_);
+}
+
+Solver::~Solver() {
+  if (st_) {
+    (void)hipStreamSynchronize(st_);
+    (void)hipStreamDestroy(st_);
+  }
+  if (hS_) (void)hipHostFree(hS_);
+}
+
+void Solver::set_params(const ds_params& p) {
+  P_ = p;
+  if (P_.refactor_freq <= 0) P_.refactor_freq = 150;
+  if (P_.refactor_freq > ETA_MAX - 8) throw std::invalid_argument("refactor_freq exceeds the eta file capacity");
+  if (P_.chunk <= 0) P_.chunk = 1;
+}
+
+DevModel Solver::M() const {
+  DevModel m;
+  m.m = H_.m;
+  m.n = H_.n;
+  m.N = H_.N;
+  m.nnz = (int)H_.ridx.size();
+  m.cptr = cptr_.p;
+  m.ridx = ridx_.p;
+  m.cval = cval_.p;
+  m.rptr = rptr_.p;
+  m.cidx = cidx_.p;
+  m.rval = rval_.p;
+  m.c = c_.p;
+  m.l = l_.p;
+  m.u = u_.p;
+  m.bt = bt_.p;
+  return m;
+}
+
+DevIter Solver::I() const {
+  DevIter it;
+  it.x = x_.p;
+  it.d = d_.p;
+  it.beta = beta_.p;
+  it.basis = basis_.p;
+  it.nonbasis = nonbasis_.p;
+  it.posN = posN_.p;
+  it.f = f_.p;
+  it.rho = rho_.p;
+  it.alpha_r = alpha_r_.p;
+  it.alpha_q = alpha_q_.p;
+  it.tau = tau_.p;
+  it.w1 = w1_.p;
+  it.w2 = w2_.p;
+  it.wN = wN_.p;
+  it.etas = etas_.p;
+  it.eta_pos = eta_pos_.p;
+  it.E = E_.p;
+  it.chain = chain_.p;
+  it.eta_cap = eta_cap_;
+  it.blocked = blocked_.p;
+  it.blocked_list = blocked_list_.p;
+  it.list_idx = list_idx_.p;
+  it.list_val = list_val_.p;
+  it.ylist_row = ylist_row_.p;
+  it.ylist_val = ylist_val_.p;
+  it.list_n = list_n_.p;
+  it.list_cap = (int)list_idx_.n;
+  it.S = S_.p;
+  it.part_d = part_d_.p;
+  it.part_i = part_i_.p;
+  it.part_cap = 2048;
+  return it;
+}
+
+DevFactor Solver::F() const {
+  DevFactor f;
+  f.k = k_;
+  f.Kinv = Kinv_.p;
+  f.KinvT = KinvT_.p;
+  f.pos_P = pos_P_.p;
+  f.row_R = row_R_.p;
+  f.Ridx_of_row = Ridx_.p;
+  f.Pidx_of_pos = Pidx_.p;
+  f.slackpos_of_row = slackpos_.p;
+  f.slackrow_of_pos = slackrow_.p;
+  f.crp = crp_.p;
+  f.cci = cci_.p;
+  f.ccv = ccv_.p;
+  f.ccp = ccp_.p;
+  f.ccr = ccr_.p;
+  f.ccv2 = ccv2_.p;
+  return f;
+}
+
+void Solver::pull() {
+  PDS_HIP(hipGetLastError());
+  PDS_HIP(hipMemcpyAsync(hS_, S_.p, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
+  PDS_HIP(hipStreamSynchronize(st_));
+}
+void Solver::push() {
+  PDS_HIP(hipMemcpyAsync(S_.p, hS_, sizeof(DevScalars), hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipStreamSynchronize(st_));
+}
+void Solver::sync() { PDS_HIP(hipStreamSynchronize(st_)); }
+
+void Solver::tick_limit(int it) {
+  if (P_.max_iterations > 0 && it > P_.max_iterations) {
+    iteration_ = P_.max_iterations;
+    throw IterLimit{};
+  }
+}
+
+template <typename T>
+static void upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+  b.alloc(v.size());
+  if (!v.empty()) PDS_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+void Solver::load(const HostModel& hm) {
+  H_ = hm;
+  const int m = H_.m, N = H_.N, n = H_.n;
+  const size_t nnz = H_.ridx.size();
+  upload(cptr_, H_.cptr, st_);
+  upload(ridx_, H_.ridx, st_);
+  upload(cval_, H_.cval, st_);
+  upload(rptr_, H_.rptr, st_);
+  upload(cidx_, H_.cidx, st_);
+  upload(rval_, H_.rval, st_);
+  upload(l_, H_.l, st_);
+  upload(u_, H_.u, st_);
+  upload(bt_, H_.bt, st_);
+  c_host_ = H_.c;
+  c_orig_ = H_.c;
+  upload(c_, c_host_, st_);
+  eta_cap_ = ETA_MAX;
+  x_.alloc(N);
+  d_.alloc(N);
+  beta_.alloc(m);
+  f_.alloc(m);
+  rho_.alloc(m);
+  alpha_r_.alloc(n);
+  alpha_q_.alloc(m);
+  tau_.alloc(m);
+  w1_.alloc(m);
+  w2_.alloc(m);
+  wN_.alloc(N);
+  etas_.alloc((size_t)eta_cap_ * m);
+  E_.alloc((size_t)eta_cap_ * eta_cap_);
+  chain_.alloc(eta_cap_);
+  eta_pos_.alloc(eta_cap_);
+  part_d_.alloc(4096);
+  part_i_.alloc(8192);
+  basis_.alloc(m);
+  nonbasis_.alloc(n);
+  posN_.alloc(N);
+  blocked_.alloc(N);
+  blocked_list_.alloc(MAX_BLOCKED);
+  list_idx_.alloc(1 << 17);
+  list_val_.alloc(1 << 17);
+  ylist_row_.alloc(ETA_MAX + 1);
+  ylist_val_.alloc(ETA_MAX + 1);
+  list_n_.alloc(2);
+  S_.alloc(1);
+  pos_P_.alloc(m);
+  row_R_.alloc(m);
+  Ridx_.alloc(m);
+  Pidx_.alloc(m);
+  slackpos_.alloc(m);
+  slackrow_.alloc(m);
+  crp_.alloc(m + 1);
+  cci_.alloc(nnz);
+  ccv_.alloc(nnz);
+  ccp_.alloc(m + 1);
+  ccr_.alloc(nnz);
+  ccv2_.alloc(nnz);
+  piv_.alloc(m);
+  perm_.alloc(m);
+  flag_.alloc(1);
+  Pf_.alloc((size_t)m * GJ_NB);
+  Pb_.alloc((size_t)m * GJ_NB);
+  PDS_HIP(hipMemsetAsync(blocked_.p, 0, N, st_));
+  PDS_HIP(hipMemsetAsync(E_.p, 0, sizeof(double) * eta_cap_ * eta_cap_, st_));
+  sync();
+  loaded_ = true;
+  inited_ = false;
+}
+
+// Kernel buffers grow with the largest kernel seen (k <= m).
+static void ensure_kernel_capacity(DBuf<double>& b, int k) {
+  size_t need = (size_t)k * (size_t)k;
+  if (need > b.n) {
+    size_t cap = std::max(need, (size_t)(b.n * 3 / 2));
+    b.alloc(cap);
+  }
+}
+
+void Solver::refactor() {
+  const double t0 = now_ms();
+  const int m = H_.m, n = H_.n;
+  std::vector<int> basis(m);
+  PDS_HIP(hipMemcpyAsync(basis.data(), basis_.p, sizeof(int) * m, hipMemcpyDeviceToHost, st_));
+  sync();
+  std::vector<int> pos_P, row_R, Ridx(m, -1), Pidx(m, -1), slackpos(m, -1), slackrow(m, -1), varpos(H_.N, -1);
+  for (int i = 0; i < m; ++i) {
+    int j = basis[i];
+    varpos[j] = i;
+    if (j >= n) {
+      slackpos[j - n] = i;
+      slackrow[i] = j - n;
+    } else {
+      Pidx[i] = (int)pos_P.size();
+      pos_P.push_back(i);
+    }
+  }
+  for (int r = 0; r < m; ++r)
+    if (slackpos[r] < 0) {
+      Ridx[r] = (int)row_R.size();
+      row_R.push_back(r);
+    }
+  const int k = (int)pos_P.size();
+  if ((int)row_R.size() != k) throw SingularError("basis is structurally singular");
+  // Coupling block C = A[covered rows, basic structurals], by rows and by kernel columns.
+  std::vector<int> crp(m + 1, 0), cci, ccp(k + 1, 0), ccr;
+  std::vector<double> ccv, ccv2;
+  for (int r = 0; r < m; ++r) {
+    if (slackpos[r] >= 0)
+      for (int e = H_.rptr[r]; e < H_.rptr[r + 1]; ++e) {
+        int j = H_.cidx[e];
+        if (j < n && varpos[j] >= 0) {
+          cci.push_back(Pidx[varpos[j]]);
+          ccv.push_back(H_.rval[e]);
+        }
+      }
+    crp[r + 1] = (int)cci.size();
+  }
+  for (int c = 0; c < k; ++c) {
+    int j = basis[pos_P[c]];
+    for (int e = H_.cptr[j]; e < H_.cptr[j + 1]; ++e) {
+      int r = H_.ridx[e];
+      if (slackpos[r] >= 0) {
+        ccr.push_back(r);
+        ccv2.push_back(H_.cval[e]);
+      }
+    }
+    ccp[c + 1] = (int)ccr.size();
+  }
+  pos_P.resize(std::max(k, 1));
+  row_R.resize(std::max(k, 1));
+  PDS_HIP(hipMemcpyAsync(pos_P_.p, pos_P.data(), sizeof(int) * pos_P.size(), hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(row_R_.p, row_R.data(), sizeof(int) * row_R.size(), hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(Ridx_.p, Ridx.data(), sizeof(int) * m, hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(Pidx_.p, Pidx.data(), sizeof(int) * m, hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(slackpos_.p, slackpos.data(), sizeof(int) * m, hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(slackrow_.p, slackrow.data(), sizeof(int) * m, hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(crp_.p, crp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st_));
+  PDS_HIP(hipMemcpyAsync(ccp_.p, ccp.data(), sizeof(int) * (k + 1), hipMemcpyHostToDevice, st_));
+  if (!cci.empty()) {
+    PDS_HIP(hipMemcpyAsync(cci_.p, cci.data(), sizeof(int) * cci.size(), hipMemcpyHostToDevice, st_));
+    PDS_HIP(hipMemcpyAsync(ccv_.p, ccv.data(), sizeof(double) * ccv.size(), hipMemcpyHostToDevice, st_));
+    PDS_HIP(hipMemcpyAsync(ccr_.p, ccr.data(), sizeof(int) * ccr.size(), hipMemcpyHostToDevice, st_));
+    PDS_HIP(hipMemcpyAsync(ccv2_.p, ccv2.data(), sizeof(double) * ccv2.size(), hipMemcpyHostToDevice, st_));
+  }
+  k_ = k;
+  if (k > 0) {
+    ensure_kernel_capacity(GW_, k);
+    ensure_kernel_capacity(Kinv_, k);
+    ensure_kernel_capacity(KinvT_, k);
+    int zero = 0;
+    PDS_HIP(hipMemcpyAsync(flag_.p, &zero, sizeof(int), hipMemcpyHostToDevice, st_));
+    launch_build_kernel(st_, M(), basis_.p, F(), GW_.p);
+    GJWork G{GW_.p, Pf_.p, Pb_.p, piv_.p, perm_.p, flag_.p, gj_mfma_};
+    launch_gj_invert(st_, k, G, Kinv_.p, KinvT_.p);
+    int flag = 0;
+    PDS_HIP(hipMemcpyAsync(&flag, flag_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+    sync();
+    if (flag) throw SingularError("singular basis kernel in refactorization");
+  }
+  pull();
+  hS_->t = 0;
+  hS_->k = k;
+  push();
+  t_ = 0;
+  refactors_++;
+  refactor_ms_ += now_ms() - t0;
+}
+
+void Solver::initBetaWeights(bool identity) {
+  if (identity) launch_fill(st_, beta_.p, 1.0, H_.m);
+  else launch_dse_exact(st_, M(), I(), F());
+}
+
+// y = B^{-T} c_B ; d_N = c_N - A_N^T y  (sequentialDualSimplex.cpp:119-132)
+void Solver::initReducedCosts() {
+  launch_gather_cb(st_, M(), I(), w1_.p);
+  launch_btran_dense(st_, M(), I(), F(), w1_.p, rho_.p, t_);
+  launch_spmv(st_, M(), posN_.p, rho_.p, -1.0, 1.0, c_.p, d_.p, 1, S_.p, 0);
+}
+
+// sequentialDualSimplex.cpp:230-250
+void Solver::dualSimplexInit() {
+  launch_set_primal(st_, M(), I());
+  launch_mask_nonbasic(st_, M(), I(), x_.p, wN_.p);
+  launch_rowspmv(st_, M(), wN_.p, nullptr, -1.0, 1.0, w1_.p);
+  launch_ftran_dense(st_, M(), I(), F(), w1_.p, x_.p, t_, 1, 0);
+  initReducedCosts();
+  launch_dot_obj(st_, M(), I());
+}
+
+// minimizeDualInfeasibilityInit (tpp:776-795) + initPhase1PricingVector
+void Solver::phase1Init() {
+  initReducedCosts();
+  launch_phase1_inf(st_, M(), I());
+  launch_rowspmv(st_, M(), wN_.p, nullptr, 1.0, 0.0, w1_.p);
+  launch_ftran_dense(st_, M(), I(), F(), w1_.p, f_.p, t_, 0, 0);
+}
+
+bool Solver::checkPerturbNeed() const {
+  std::vector<double> cs(c_host_);
+  std::sort(cs.begin(), cs.end());
+  int uniq = 0;
+  double last = 0.0;
+  for (size_t i = 0; i < cs.size(); ++i)
+    if (i == 0 || !(std::fabs(last - cs[i]) < EPS_COSTS)) {
+      last = cs[i];
+      ++uniq;
+    }
+  return uniq < PERTURB_RATIO * (double)cs.size();
+}
+
+// perturbCosts (tpp:238-275) with a seeded stream in place of random_device.
+void Solver::perturbCosts() {
+  static const double wts[10] = {0.01, 0.1, 1, 2, 5, 10, 20, 30, 40, 100};
+  SplitMix rng(P_.perturb_seed);
+  rng.s = rng_state_ ? rng_state_ : P_.perturb_seed;
+  c_orig_ = c_host_;
+  double mean = 0.0;
+  for (double v : c_host_) mean = mean + v;
+  mean = mean / (double)c_host_.size();
+  const double min_p = std::min(1e-2 * EPS_D, PSI);
+  const double max_p = std::max(1e+3 * EPS_D, PSI * 10 * mean);
+  for (int i = 0; i < H_.N; ++i) {
+    double magnitude = 100 * EPS_D + PSI * c_host_[i];
+    double pert;
+    if (H_.bt[i] != BT_LOWER && H_.bt[i] != BT_FREE) pert = -0.5 * magnitude * (1 + rng.uniform());
+    pert = 0.5 * magnitude * (1 + rng.uniform());
+    int cnt = H_.cptr[i + 1] - H_.cptr[i];
+    int kk = cnt > 10 ? 9 : (cnt == 0 ? 0 : cnt - 1);
+    pert = wts[kk] * pert;
+    if (pert != 0.0)
+      while (std::fabs(pert) > std::fabs(max_p) || std::fabs(pert) < std::fabs(min_p)) {
+        if (std::fabs(pert) > std::fabs(max_p)) pert = 0.1 * pert;
+        else if (std::fabs(pert) < std::fabs(min_p)) pert = 10 * pert;
+      }
+    c_host_[i] += pert;
+  }
+  rng_state_ = rng.s;
+  perturbed_ = true;
+  upload_costs();
+  pull();
+  hS_->perturbed = 1;
+  push();
+}
+
+void Solver::upload_costs() {
+  PDS_HIP(hipMemcpyAsync(c_.p, c_host_.data(), sizeof(double) * H_.N, hipMemcpyHostToDevice, st_));
+  sync();
+}
+
+void Solver::initDualSimplex() {
+  if (!loaded_) throw std::logic_error("no problem loaded");
+  const int m = H_.m, n = H_.n, N = H_.N;
+  std::vector<int> basis(m), nonbasis(n), posN(N, -1);
+  for (int i = 0; i < n; ++i) {
+    nonbasis[i] = i;
+    posN[i] = i;
+  }
+  for (int i = 0; i < m; ++i) basis[i] = n + i;
+  upload(basis_, basis, st_);
+  upload(nonbasis_, nonbasis, st_);
+  upload(posN_, posN, st_);
+  launch_fill(st_, x_.p, 0.0, N);
+  launch_fill(st_, d_.p, 0.0, N);
+  launch_fill(st_, beta_.p, 1.0, m);
+  std::memset(hS_, 0, sizeof(DevScalars));
+  push();
+  c_host_ = H_.c;
+  c_orig_ = H_.c;
+  upload_costs();
+  perturbed_ = false;
+  rng_state_ = 0;
+  iteration_ = 0;
+  t_ = 0;
+  refactors_ = restarts_ = phase1_iters_ = 0;
+  solve_ms_ = refactor_ms_ = 0.0;
+  status_ = DS_NOT_SOLVED;
+  solved_ = false;
+  message_.clear();
+  refactor();
+  inited_ = true;
+}
+
+void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t) {
+  const DevModel m = M();
+  const DevIter i = I();
+  const DevFactor f = F();
+  if (phase == 2 && dualcheck) launch_dualcheck(st_, m, i, it);
+  launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
+  launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
+  launch_chuzc(st_, m, i, phase, P_.ratio_test, it);
+  launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
+  launch_dupdate(st_, m, i, phase, it);
+  enqueue_stageB(phase, it, t);
+}
+
+void Solver::enqueue_stageB(int phase, int it, int t) {
+  launch_ftran_dense(st_, M(), I(), F(), rho_.p, tau_.p, t, 0, 1);
+  launch_update(st_, M(), I(), phase, it);
+}
+
+// Queue iterations next, next+1, ... up to the chunk size, stopping before
+// the next scheduled refactor and at the iteration cap.
+int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
+  const int K = P_.refactor_freq;
+  int L = 0;
+  for (int i = next;; ++i) {
+    if (L > 0) {
+      if (i % K == 0 || L >= P_.chunk) break;
+      if (P_.max_iterations > 0 && i > P_.max_iterations) break;
+    }
+    if (t + L + 1 >= eta_cap_) break;
+    bool dc = (phase == 2) && (i % NEED_RESTART == 0) && (L > 0 || skip_first == 0);
+    enqueue_body(phase, i, L == 0 ? skip_first : 0, dc, t + L);
+    ++L;
+  }
+  return L;
+}
+
+// minimizeDualInfeasibility (baseDualSimplex.tpp:799-934).
+int Solver::phase1() {
+  const int K = P_.refactor_freq;
+  phase1Init();
+  pull();
+  hS_->cycle = 0;
+  hS_->status = ST_RUNNING;
+  push();
+  int resume = -1, resume_skip = 0;
+  while (true) {
+    int next, skip_first = 0;
+    if (resume >= 0) {
+      next = resume;
+      skip_first = resume_skip;
+      resume = -1;
+    } else {
+      next = iteration_ + 1;
+      tick_limit(next);
+      if (next % K == 0) {
+        // Top-of-iteration checks in the reference's order, then the refactor.
+        pull();
+        if (P_.perturb && !hS_->perturbed && hS_->cycle > MAX_CYCLE) {
+          perturbCosts();
+          phase1Init();
+          pull();
+        }
+        if (hS_->cycle > NEED_RESTART || hS_->obj > BIG || std::isnan(hS_->obj)) {
+          phase1Init();
+        }
+        refactor();
+        skip_first = 1;
+      }
+    }
+    int L = plan_chunk(1, next, t_, skip_first);
+    pull();
+    if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
+      iteration_ = next + L - 1;
+      t_ = hS_->t;
+      continue;
+    }
+    const int stop = hS_->stop_iter;
+    iteration_ = stop;
+    t_ = hS_->t;
+    if (hS_->refac_mid && hS_->status == ST_RUNNING) {
+      hS_->refac_mid = 0;
+      push();
+      refactor();
+      enqueue_stageB(1, stop, 0);
+      pull();
+      t_ = hS_->t;
+      continue;
+    }
+    const int st = hS_->status;
+    hS_->status = ST_RUNNING;
+    hS_->refac_mid = 0;
+    push();
+    switch (st) {
+      case ST_PH1_SOLVED:
+        return 0;
+      case ST_PH1_DUALINF:
+        solved_ = false;
+        message_ = "dual infeasible";
+        return 1;
+      case ST_NEED_PERTURB:
+        perturbCosts();
+        phase1Init();
+        pull();
+        if (hS_->cycle > NEED_RESTART || hS_->obj > BIG || std::isnan(hS_->obj)) phase1Init();
+        resume = stop;
+        resume_skip = 1;
+        break;
+      case ST_PH1_REINIT:
+        phase1Init();
+        resume = stop;
+        resume_skip = 1;
+        break;
+      case ST_PH1_RESTART: {
+        restarts_++;
+        refactor();
+        phase1Init();
+        initBetaWeights(false);
+        pull();
+        std::vector<int> lst(hS_->nblocked);
+        if (!lst.empty()) {
+          PDS_HIP(hipMemcpy(lst.data(), blocked_list_.p, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+          unsigned char z = 0;
+          for (int j : lst) PDS_HIP(hipMemcpy(blocked_.p + j, &z, 1, hipMemcpyHostToDevice));
+        }
+        hS_->nblocked = 0;
+        hS_->any_blocked = 0;
+        push();
+        break;
+      }
+      case ST_NO_RATIO: {
+        restarts_++;
+        pull();
+        int p = hS_->p;
+        unsigned char one = 1;
+        PDS_HIP(hipMemcpy(blocked_.p + p, &one, 1, hipMemcpyHostToDevice));
+        if (hS_->nblocked < MAX_BLOCKED) {
+          PDS_HIP(hipMemcpy(blocked_list_.p + hS_->nblocked, &p, sizeof(int), hipMemcpyHostToDevice));
+          hS_->nblocked++;
+        }
+        hS_->any_blocked = 1;
+        push();
+        refactor();
+        phase1Init();
+        break;
+      }
+      case ST_BTRAN_CAP:
+        throw std::runtime_error("sparse BTRAN list capacity exceeded");
+      default:
+        throw std::runtime_error("unexpected device status " + std::to_string(st) + " in phase 1");
+    }
+  }
+}
+
+// callDualSolver (sequentialDualSimplex.cpp:253-515).
+bool Solver::phase2() {
+  const int K = P_.refactor_freq;
+  dualSimplexInit();
+  pull();
+  hS_->cycle = 0;
+  hS_->unbound_checked = 0;
+  hS_->status = ST_RUNNING;
+  push();
+  int resume = -1, resume_skip = 0;
+  while (true) {
+    int next, skip_first = 0;
+    if (resume >= 0) {
+      next = resume;
+      skip_first = resume_skip;
+      resume = -1;
+    } else {
+      next = iteration_ + 1;
+      tick_limit(next);
+      if (next % K == 0) refactor();
+    }
+    int L = plan_chunk(2, next, t_, skip_first);
+    pull();
+    if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
+      iteration_ = next + L - 1;
+      t_ = hS_->t;
+      continue;
+    }
+    const int stop = hS_->stop_iter;
+    iteration_ = stop;
+    t_ = hS_->t;
+    if (hS_->status == ST_BFRT) throw Unsupported("bound flipping of boxed variables is not supported on the GPU path");
+    if (hS_->refac_mid && hS_->status == ST_RUNNING) {
+      hS_->refac_mid = 0;
+      push();
+      refactor();
+      enqueue_stageB(2, stop, 0);
+      pull();
+      t_ = hS_->t;
+      continue;
+    }
+    const int st = hS_->status;
+    hS_->status = ST_RUNNING;
+    hS_->refac_mid = 0;
+    push();
+    switch (st) {
+      case ST_OPTIMAL:
+        solved_ = true;
+        status_ = DS_OPTIMAL;
+        message_ = "optimal solution";
+        return true;
+      case ST_NO_PIVOT:
+        restarts_++;
+        refactor();
+        initBetaWeights(false);
+        dualSimplexInit();
+        break;
+      case ST_DUAL_RESTART: {
+        restarts_++;
+        refactor();
+        pull();
+        int cyc = hS_->cycle;
+        phase1();
+        dualSimplexInit();
+        pull();
+        hS_->cycle = cyc;
+        push();
+        resume = iteration_;
+        resume_skip = 0;
+        break;
+      }
+      case ST_NEED_PERTURB:
+        perturbCosts();
+        refactor();
+        dualSimplexInit();
+        resume = stop;
+        resume_skip = 1;
+        break;
+      case ST_NO_RATIO: {
+        pull();
+        if (hS_->unbound_checked) {
+          solved_ = false;
+          status_ = DS_PRIMAL_INFEASIBLE;
+          message_ = "dual unbounded";
+          return false;
+        }
+        restarts_++;
+        hS_->unbound_checked = 1;
+        push();
+        refactor();
+        initBetaWeights(false);
+        pull();
+        int cyc = hS_->cycle;
+        phase1();
+        dualSimplexInit();
+        pull();
+        hS_->cycle = cyc;
+        hS_->unbound_checked = 1;
+        push();
+        break;
+      }
+      case ST_BTRAN_CAP:
+        throw std::runtime_error("sparse BTRAN list capacity exceeded");
+      default:
+        throw std::runtime_error("unexpected device status " + std::to_string(st) + " in phase 2");
+    }
+  }
+}
+
+int Solver::presolve() {
+  if (!inited_) throw std::logic_error("initDualSimplex not called");
+  const double t0 = now_ms();
+  int st = 0;
+  try {
+    if (P_.perturb && checkPerturbNeed()) perturbCosts();
+    initBetaWeights(true);
+    st = phase1();
+    phase1_iters_ = iteration_;
+    if (st == 1) {
+      status_ = DS_DUAL_INFEASIBLE;
+      solved_ = false;
+      message_ = "dual infeasible";
+    } else {
+      launch_set_primal(st_, M(), I());
+    }
+    sync();
+  } catch (const IterLimit&) {
+    status_ = DS_ITERATION_LIMIT;
+    message_ = "iteration limit";
+    st = 2;
+  }
+  solve_ms_ += now_ms() - t0;
+  return st;
+}
+
+void Solver::solve() {
+  const double t0 = now_ms();
+  if (status_ == DS_ITERATION_LIMIT) return;
+  try {
+    phase2();
+    // postProcess (tpp:123-159): restore the original costs and recompute.
+    if (perturbed_) {
+      c_host_ = c_orig_;
+      upload_costs();
+      dualSimplexInit();
+    }
+    pull();
+    Z_ = hS_->obj;
+  } catch (const IterLimit&) {
+    status_ = DS_ITERATION_LIMIT;
+    message_ = "iteration limit";
+    pull();
+    Z_ = hS_->obj;
+  }
+  solve_ms_ += now_ms() - t0;
+}
+
+void Solver::result(ds_result& r) const {
+  r.status = status_;
+  r.solved = solved_ ? 1 : 0;
+  r.iterations = iteration_;
+  r.phase1_iterations = phase1_iters_;
+  r.perturbed = perturbed_ ? 1 : 0;
+  r.refactors = refactors_;
+  r.restarts = restarts_;
+  r.Z = Z_ - H_.offset;
+  r.solve_ms = solve_ms_;
+  r.refactor_ms = refactor_ms_;
+}
+
+void Solver::solution(double& Z, int& solved, std::string& message, std::vector<double>& x) {
+  x.resize(H_.N);
+  PDS_HIP(hipMemcpyAsync(x.data(), x_.p, sizeof(double) * H_.N, hipMemcpyDeviceToHost, st_));
+  sync();
+  Z = Z_ - H_.offset;
+  solved = solved_ ? 1 : 0;
+  message = message_;
+}
+
+std::vector<int> Solver::basis_host() {
+  std::vector<int> b(H_.m);
+  PDS_HIP(hipMemcpyAsync(b.data(), basis_.p, sizeof(int) * H_.m, hipMemcpyDeviceToHost, st_));
+  sync();
+  return b;
+}
+
+std::vector<int> Solver::nonbasis_host() {
+  std::vector<int> b(H_.n);
+  PDS_HIP(hipMemcpyAsync(b.data(), nonbasis_.p, sizeof(int) * H_.n, hipMemcpyDeviceToHost, st_));
+  sync();
+  return b;
+}
+
+void Solver::set_basis(const int* basis) {
+  if (!inited_) initDualSimplex();
+  const int m = H_.m, n = H_.n, N = H_.N;
+  std::vector<int> b(basis, basis + m), nb, posN(N, -1);
+  std::vector<char> isb(N, 0);
+  for (int i = 0; i < m; ++i) {
+    if (b[i] < 0 || b[i] >= N || isb[b[i]]) throw std::invalid_argument("invalid basis");
+    isb[b[i]] = 1;
+  }
+  for (int j = 0; j < N; ++j)
+    if (!isb[j]) {
+      posN[j] = (int)nb.size();
+      nb.push_back(j);
+    }
+  (void)n;
+  upload(basis_, b, st_);
+  upload(nonbasis_, nb, st_);
+  upload(posN_, posN, st_);
+  sync();
+  refactor();
+}
+
+void Solver::btran(int p_idx, double* rho) {
+  if (p_idx < 0 || p_idx >= H_.m) throw std::invalid_argument("p_idx out of range");
+  launch_btran_unit(st_, M(), I(), F(), t_, rho_.p, p_idx);
+  PDS_HIP(hipMemcpyAsync(rho, rho_.p, sizeof(double) * H_.m, hipMemcpyDeviceToHost, st_));
+  sync();
+}
+
+void Solver::ftran(const double* b, double* x) {
+  PDS_HIP(hipMemcpyAsync(w1_.p, b, sizeof(double) * H_.m, hipMemcpyHostToDevice, st_));
+  launch_ftran_dense(st_, M(), I(), F(), w1_.p, tau_.p, t_, 0, 0);
+  PDS_HIP(hipMemcpyAsync(x, tau_.p, sizeof(double) * H_.m, hipMemcpyDeviceToHost, st_));
+  sync();
+}
+
+void Solver::ftran_column(int j, double* x) {
+  if (j < 0 || j >= H_.N) throw std::invalid_argument("column out of range");
+  launch_ftran_col(st_, M(), I(), F(), j, t_, alpha_q_.p, 0);
+  PDS_HIP(hipMemcpyAsync(x, alpha_q_.p, sizeof(double) * H_.m, hipMemcpyDeviceToHost, st_));
+  sync();
+}
+
+void Solver::pivot_row(const double* rho, double* alpha) {
+  PDS_HIP(hipMemcpyAsync(rho_.p, rho, sizeof(double) * H_.m, hipMemcpyHostToDevice, st_));
+  launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0);
+  PDS_HIP(hipMemcpyAsync(alpha, alpha_r_.p, sizeof(double) * H_.n, hipMemcpyDeviceToHost, st_));
+  sync();
+}
+
+void Solver::time_pivot_row(const double* rho, int reps, double& ms, double& bytes) {
+  PDS_HIP(hipMemcpyAsync(rho_.p, rho, sizeof(double) * H_.m, hipMemcpyHostToDevice, st_));
+  launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0);
+  hipEvent_t e0, e1;
+  PDS_HIP(hipEventCreate(&e0));
+  PDS_HIP(hipEventCreate(&e1));
+  PDS_HIP(hipEventRecord(e0, st_));
+  for (int r = 0; r < reps; ++r) launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0);
+  PDS_HIP(hipEventRecord(e1, st_));
+  PDS_HIP(hipEventSynchronize(e1));
+  float total = 0.f;
+  PDS_HIP(hipEventElapsedTime(&total, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ms = reps > 0 ? (double)total / reps : 0.0;
+  // Algorithmic bytes: 12 per nonzero of the nonbasic columns, their extents,
+  // the nonbasic index list, rho once and alpha written once.
+  std::vector<int> nb = nonbasis_host();
+  double nnzN = 0;
+  for (int j : nb) nnzN += H_.cptr[j + 1] - H_.cptr[j];
+  bytes = 12.0 * nnzN + 4.0 * (nb.size() + 1) + 8.0 * nb.size() + 8.0 * H_.m;
+}
+
+}  // namespace pds

@@ -1,0 +1,155 @@
+// Host side of the MI355X dual simplex: the LP model in computational form
+// (the reference's Problem, common/problem.{hpp,tpp}) and the solver driving
+// the gfx950 kernels (the reference's SequentialDualSimplex /
+// BaseDualSimplex control flow, cpu/sequentialDualSimplex.cpp and
+// common/baseDualSimplex.tpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ds_abi.h"
+#include "ds_kernels.hpp"
+
+namespace pds {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct SingularError : std::runtime_error {
+  explicit SingularError(const std::string& s) : std::runtime_error(s) {}
+};
+struct Unsupported : std::runtime_error {
+  explicit Unsupported(const std::string& s) : std::runtime_error(s) {}
+};
+struct IterLimit {};
+
+#define PDS_HIP(x)                                                                                      \
+  do {                                                                                                  \
+    hipError_t e_ = (x);                                                                                \
+    if (e_ != hipSuccess)                                                                               \
+      throw ::pds::HipError(std::string(#x) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + ":" + \
+                            std::to_string(__LINE__));                                                  \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) count = 1;
+    PDS_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// LP in computational form: A <- [A | I], slack bounds from row ranges,
+// RHS = 0 (problem.tpp:176-226).
+struct HostModel {
+  int m = 0, n = 0, N = 0;
+  std::vector<int> cptr, ridx, rptr, cidx;
+  std::vector<double> cval, rval, c, l, u;
+  std::vector<int8_t> bt;
+  double offset = 0.0;
+  void build(int m, int n, const int* col_ptr, const int* row_idx, const double* val, const double* costs,
+             const double* lb, const double* ub, const int* bound_type, const double* lr, const double* ur,
+             const int* range_type);
+};
+
+class Solver {
+ public:
+  explicit Solver(int device);
+  ~Solver();
+
+  void load(const HostModel& hm);
+  void set_params(const ds_params& p);
+  void initDualSimplex();
+  int presolve();      // 0 = Solved, 1 = DualInfeas (Phase1OutStatus)
+  void solve();        // phase 2 + postProcess
+  void result(ds_result& r) const;
+  void solution(double& Z, int& solved, std::string& message, std::vector<double>& x);
+  std::vector<int> basis_host();
+  std::vector<int> nonbasis_host();
+
+  // hook-level entries
+  void set_basis(const int* basis);
+  void refactor();
+  void btran(int p_idx, double* rho);
+  void ftran(const double* b, double* x);
+  void ftran_column(int j, double* x);
+  void pivot_row(const double* rho, double* alpha);
+  void time_pivot_row(const double* rho, int reps, double& ms, double& bytes);
+
+  std::string last_error;
+
+ private:
+  int device_ = 0;
+  hipStream_t st_ = nullptr;
+  ds_params P_{};
+  HostModel H_;
+  bool loaded_ = false, inited_ = false;
+
+  // model
+  DBuf<int> cptr_, ridx_, rptr_, cidx_;
+  DBuf<double> cval_, rval_, c_, l_, u_;
+  DBuf<int8_t> bt_;
+  // iterate
+  DBuf<double> x_, d_, beta_, f_, rho_, alpha_r_, alpha_q_, tau_, w1_, w2_, wN_, etas_, E_, chain_, part_d_,
+      list_val_, ylist_val_;
+  DBuf<int> basis_, nonbasis_, posN_, eta_pos_, blocked_list_, list_idx_, ylist_row_, list_n_, part_i_;
+  DBuf<unsigned char> blocked_;
+  DBuf<DevScalars> S_;
+  DevScalars* hS_ = nullptr;  // pinned mirror
+  // factor
+  DBuf<double> Kinv_, KinvT_, GW_, Pf_, Pb_, ccv_, ccv2_;
+  DBuf<int> pos_P_, row_R_, Ridx_, Pidx_, slackpos_, slackrow_, crp_, cci_, ccp_, ccr_, piv_, perm_, flag_;
+  int k_ = 0;
+  int eta_cap_ = 0;
+  int gj_mfma_ = 1;  // trailing update of the inversion on FP64 MFMA
+
+  // host mirrors of the control state
+  std::vector<double> c_host_, c_orig_;
+  bool perturbed_ = false;
+  int iteration_ = 0;
+  int t_ = 0;  // etas since the last refactor
+  int status_ = DS_NOT_SOLVED;
+  bool solved_ = false;
+  std::string message_;
+  double Z_ = 0.0;
+  int refactors_ = 0, restarts_ = 0, phase1_iters_ = 0;
+  double solve_ms_ = 0.0, refactor_ms_ = 0.0;
+  uint64_t rng_state_ = 0;
+
+  DevModel M() const;
+  DevIter I() const;
+  DevFactor F() const;
+
+  void pull();
+  void push();
+  void sync();
+  void tick_limit(int it);
+
+  void initBetaWeights(bool identity);
+  void initReducedCosts();
+  void dualSimplexInit();
+  void phase1Init();
+  bool checkPerturbNeed() const;
+  void perturbCosts();
+  void upload_costs();
+  int phase1();
+  bool phase2();
+  void enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t);
+  void enqueue_stageB(int phase, int it, int t);
+  int plan_chunk(int phase, int next, int t, int skip_first);
+};
+
+}  // namespace pds

@@ -1,0 +1,105 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) — test-side only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+
+class OraParams(C.Structure):
+    _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
+                ("tie_rule", C.c_int), ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("verbose", C.c_int)]
+
+
+class OraResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("solved", C.c_int), ("iterations", C.c_int), ("perturbed", C.c_int),
+                ("refactors", C.c_int), ("phase1_restarts", C.c_int), ("Z", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+        L = C.CDLL(ORACLE_LIB)
+        vp = C.c_void_p
+        L.ora_solve.restype = C.c_int
+        L.ora_solve.argtypes = [C.c_int, C.c_int] + [vp] * 10 + [C.POINTER(OraParams), C.POINTER(OraResult), vp, vp]
+        L.ora_dot_update.restype = C.c_int
+        L.ora_dot_update.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, vp, C.c_int,
+                                     C.c_int, C.c_int]
+        L.ora_dense_solve.restype = C.c_int
+        L.ora_dense_solve.argtypes = [C.c_int, vp, vp, vp, C.c_int]
+        L.ora_pfi_solve.restype = C.c_int
+        L.ora_pfi_solve.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, C.c_int]
+        L.ora_simple_ratio_test.restype = C.c_int
+        L.ora_simple_ratio_test.argtypes = [C.c_int, vp, vp, vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def P(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def params(refactor_freq=150, max_iterations=0, ratio_test=0, tie_rule=0, perturb=1, seed=0x5eed, verbose=0):
+    return OraParams(refactor_freq, max_iterations, ratio_test, tie_rule, perturb, seed, verbose)
+
+
+def solve(lp, **kw):
+    """Oracle solve of an LP (parallel_ds_amd.lp.LP); returns (OraResult, x[n+m], basis[m])."""
+    prm = params(**kw)
+    res = OraResult()
+    x = np.zeros(lp.n + lp.m)
+    basis = np.zeros(lp.m, np.int32)
+    rc = lib().ora_solve(lp.m, lp.n, *[P(a) for a in lp.arrays()], C.byref(prm), C.byref(res), P(x), P(basis))
+    if rc != 0:
+        raise RuntimeError("ora_solve failed")
+    return res, x, basis
+
+
+def dot_update(A_csr, vec1, vec2, alpha, beta, cols, method, set_flag, sol_len):
+    vals, row_ptr, col_idx, m, ncols = A_csr
+    vals = np.ascontiguousarray(vals, np.float64)
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col_idx = np.ascontiguousarray(col_idx, np.int32)
+    vec1 = np.ascontiguousarray(vec1, np.float64)
+    vec2 = np.ascontiguousarray(vec2, np.float64)
+    cols = np.ascontiguousarray(cols, np.int32)
+    sol = np.zeros(sol_len)
+    rc = lib().ora_dot_update(m, ncols, P(row_ptr), P(col_idx), P(vals), P(vec1), P(vec2), P(sol), alpha, beta,
+                              P(cols), len(cols), method, set_flag)
+    assert rc == 0
+    return sol
+
+
+def dense_solve(A, b, transpose):
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(len(b))
+    rc = lib().ora_dense_solve(A.shape[0], P(A), P(b), P(x), 1 if transpose else 0)
+    assert rc == 0
+    return x
+
+
+def pfi_solve(etas, piv, b, transpose):
+    etas = np.ascontiguousarray(etas, np.float64)
+    piv = np.ascontiguousarray(piv, np.int32)
+    b = np.ascontiguousarray(b, np.float64)
+    sol = np.zeros(len(b))
+    lib().ora_pfi_solve(len(b), len(piv), P(etas), P(piv), P(b), P(sol), 1 if transpose else 0)
+    return sol
+
+
+def ratio_test(F, d_of_pos, alpha, tie_rule=0):
+    F = np.ascontiguousarray(F, np.int32)
+    d_of_pos = np.ascontiguousarray(d_of_pos, np.float64)
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    return lib().ora_simple_ratio_test(len(F), P(F), P(d_of_pos), P(alpha), tie_rule)
