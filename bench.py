@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X dual simplex hot path.
+
+Workload (default): the single large sparse LP, FP64, m=20,000 rows, n=40,000
+columns, exactly 10 nonzeros per column at uniform-random rows, values U(0.1,1),
+b = A x0 + U(0,1), c = -U(0,1), refactor every 64 iterations with PFI etas in
+between (seed 1).  A step is one dual simplex iteration (pricing, BTRAN,
+pivot-row SpMV, ratio test, FTRAN, updates, periodic refactor) from the slack
+basis, phase 1 and phase 2 on the shared iteration counter.  --warmup W
+iterations run untimed, then exactly --steps K iterations are timed between
+two device synchronisations; inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0) with the roofline of the pivot-row SpMV measured
+with HIP events on the solver's stream over the timed iterations, and the CPU
+baseline: the oracle (a C++ restatement of the reference's sequential solver)
+timed on the same instance over the same iteration window, single thread.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel err≤1e-9"
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(lp, refactor_freq, warmup, steps):
+    """Oracle ms/iter over iterations (warmup, warmup+steps] of the same solve, 1 thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as ora  # test-side binding of the oracle
+    t0 = time.perf_counter()
+    ora.solve(lp, refactor_freq=refactor_freq, max_iterations=warmup)
+    t1 = time.perf_counter()
+    res, _, _ = ora.solve(lp, refactor_freq=refactor_freq, max_iterations=warmup + steps)
+    t2 = time.perf_counter()
+    window = (t2 - t1) - (t1 - t0)
+    done = min(res.iterations, warmup + steps) - warmup
+    return {"value": round(done / window, 3) if window > 0 else None, "unit": "iter/s",
+            "ms_per_iter": round(1000.0 * window / max(done, 1), 4), "cores": 1, "kind": "port",
+            "sample": "oracle on the same instance, iterations %d..%d from the slack basis "
+                      "(time of a %d-iteration run minus a %d-iteration run), 1 thread, %s"
+                      % (warmup + 1, warmup + done, warmup + steps, warmup, cpu_model())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--m", type=int, default=20000)
+    ap.add_argument("--n", type=int, default=40000)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--refactor", type=int, default=64)
+    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    from parallel_ds_amd import DualSimplex, default_params
+    from parallel_ds_amd import lp as lpmod
+
+    lp = lpmod.config1(seed=args.seed, m=args.m, n=args.n, k=args.k)
+    prm = default_params(refactor_freq=args.refactor, chunk=args.chunk,
+                         max_iterations=args.warmup + args.steps,
+                         mark_iteration=args.warmup, time_kernels=1)
+    with DualSimplex(lp, prm) as s:
+        s.initDualSimplex()
+        st = s.presolve("minInfeas")
+        if st == 0:
+            s.solve()
+        r = s.result()
+    if r.iterations < args.warmup + args.steps and r.status != 3:
+        # solved before the window ended: time what ran after the mark
+        timed = r.iterations - args.warmup
+    else:
+        timed = args.steps
+    window_ms = r.solve_ms - r.mark_ms
+    ms_per_iter = window_ms / max(timed, 1)
+    spmv_avg_ms = r.spmv_ms / max(r.spmv_launches, 1)
+    n_nonbasic = lp.n
+    nnzN = 0.5 * (r.nnz_nonbasic_mark + r.nnz_nonbasic_end)
+    spmv_bytes = 12.0 * nnzN + 4.0 * (n_nonbasic + 1) + 8.0 * n_nonbasic + 8.0 * lp.m
+    achieved = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": round(timed / (window_ms * 1e-3), 3),
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": timed,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_iter, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 LP generator)",
+        "config": {"workload": "single sparse LP m=%d n=%d, %d nnz/col, refactor every %d, iterations %d..%d "
+                               "from the slack basis" % (lp.m, lp.n, args.k, args.refactor, args.warmup + 1,
+                                                         args.warmup + timed),
+                   "m": lp.m, "n": lp.n, "nnz": lp.nnz, "seed": args.seed, "refactor_freq": args.refactor,
+                   "chunk": args.chunk, "status_at_end": r.status, "kernel_size_at_end": r.kernel_k},
+        "roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
+                     "launches": r.spmv_launches},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(lp, args.refactor, args.warmup, timed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,8 @@ typedef struct {
   uint64_t perturb_seed;  /* seed of the cost perturbation stream */
   int chunk;              /* iterations queued per host synchronisation (>= 1) */
   int verbose;
+  int mark_iteration;     /* > 0: synchronise after this iteration and record the time (ds_result.mark_ms) */
+  int time_kernels;       /* 1: time every pivot-row SpMV launch after the mark with HIP events */
 } ds_params;
 
 typedef struct {
@@ -80,6 +82,12 @@ typedef struct {
   double Z;               /* objective (offset removed) */
   double solve_ms;        /* device-resident solve wall time (phase 1 + phase 2) */
   double refactor_ms;     /* part of solve_ms spent in refactorization */
+  double mark_ms;         /* solve time up to the end of iteration mark_iteration */
+  double spmv_ms;         /* summed event time of the timed pivot-row SpMV launches */
+  int spmv_launches;
+  int kernel_k;           /* basis kernel size at the last refactor */
+  double nnz_nonbasic_mark; /* nnz of the nonbasic columns at the mark and at the end */
+  double nnz_nonbasic_end;
 } ds_result;
 
 void ds_default_params(ds_params* p);

@@ -17,13 +17,16 @@ ERRORS = {0: "DS_OK", -1: "DS_EINVAL", -2: "DS_EHIP", -3: "DS_ESINGULAR", -4: "D
 
 class Params(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
-                ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int)]
+                ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
+                ("mark_iteration", C.c_int), ("time_kernels", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("solved", C.c_int), ("iterations", C.c_int),
                 ("phase1_iterations", C.c_int), ("perturbed", C.c_int), ("refactors", C.c_int),
-                ("restarts", C.c_int), ("Z", C.c_double), ("solve_ms", C.c_double), ("refactor_ms", C.c_double)]
+                ("restarts", C.c_int), ("Z", C.c_double), ("solve_ms", C.c_double), ("refactor_ms", C.c_double),
+                ("mark_ms", C.c_double), ("spmv_ms", C.c_double), ("spmv_launches", C.c_int), ("kernel_k", C.c_int),
+                ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double)]
 
 
 class DsError(RuntimeError):

@@ -57,6 +57,8 @@ void ds_default_params(ds_params* p) {
   p->perturb_seed = 0x5eedULL;
   p->chunk = 32;
   p->verbose = 0;
+  p->mark_iteration = 0;
+  p->time_kernels = 0;
 }
 
 int ds_create(int device, ds_ctx** out) {

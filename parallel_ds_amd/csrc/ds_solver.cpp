@@ -95,6 +95,7 @@ Solver::Solver(int device) : device_(device) {
 }
 
 Solver::~Solver() {
+  for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
   if (st_) {
     (void)hipStreamSynchronize(st_);
     (void)hipStreamDestroy(st_);
@@ -107,6 +108,42 @@ void Solver::set_params(const ds_params& p) {
   if (P_.refactor_freq <= 0) P_.refactor_freq = 150;
   if (P_.refactor_freq > ETA_MAX - 8) throw std::invalid_argument("refactor_freq exceeds the eta file capacity");
   if (P_.chunk <= 0) P_.chunk = 1;
+  if (P_.time_kernels && (int)ev_.size() < 2 * P_.chunk) {
+    for (int i = (int)ev_.size(); i < 2 * P_.chunk; ++i) {
+      hipEvent_t e;
+      PDS_HIP(hipEventCreate(&e));
+      ev_.push_back(e);
+    }
+    ev_iter_.assign(P_.chunk, 0);
+  }
+}
+
+// Sum the SpMV event times of the queued iterations that ran (<= last_done).
+void Solver::harvest_events(int last_done) {
+  if (!P_.time_kernels) return;
+  for (int i = 0; i < ev_used_; ++i) {
+    if (ev_iter_[i] > last_done) continue;
+    float ms = 0.f;
+    PDS_HIP(hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]));
+    spmv_ms_ += ms;
+    spmv_launches_++;
+  }
+  ev_used_ = 0;
+}
+
+double Solver::nnz_nonbasic() {
+  std::vector<int> nb = nonbasis_host();
+  double s = 0.0;
+  for (int j : nb) s += H_.cptr[j + 1] - H_.cptr[j];
+  return s;
+}
+
+void Solver::maybe_mark(int done) {
+  if (P_.mark_iteration > 0 && !marked_ && done >= P_.mark_iteration) {
+    marked_ = true;
+    mark_ms_ = now_ms() - t_start_;
+    nnzN_mark_ = nnz_nonbasic();
+  }
 }
 
 DevModel Solver::M() const {
@@ -484,6 +521,11 @@ void Solver::initDualSimplex() {
   t_ = 0;
   refactors_ = restarts_ = phase1_iters_ = 0;
   solve_ms_ = refactor_ms_ = 0.0;
+  mark_ms_ = spmv_ms_ = 0.0;
+  spmv_launches_ = 0;
+  marked_ = false;
+  ev_used_ = 0;
+  nnzN_mark_ = nnzN_end_ = 0.0;
   status_ = DS_NOT_SOLVED;
   solved_ = false;
   message_.clear();
@@ -498,7 +540,13 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   if (phase == 2 && dualcheck) launch_dualcheck(st_, m, i, it);
   launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
   launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  const bool timed = P_.time_kernels && marked_ && ev_used_ < (int)ev_iter_.size();
+  if (timed) PDS_HIP(hipEventRecord(ev_[2 * ev_used_], st_));
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
+  if (timed) {
+    PDS_HIP(hipEventRecord(ev_[2 * ev_used_ + 1], st_));
+    ev_iter_[ev_used_++] = it;
+  }
   launch_chuzc(st_, m, i, phase, P_.ratio_test, it);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
   launch_dupdate(st_, m, i, phase, it);
@@ -519,6 +567,7 @@ int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
     if (L > 0) {
       if (i % K == 0 || L >= P_.chunk) break;
       if (P_.max_iterations > 0 && i > P_.max_iterations) break;
+      if (!marked_ && P_.mark_iteration > 0 && i == P_.mark_iteration + 1) break;
     }
     if (t + L + 1 >= eta_cap_) break;
     bool dc = (phase == 2) && (i % NEED_RESTART == 0) && (L > 0 || skip_first == 0);
@@ -566,9 +615,12 @@ int Solver::phase1() {
     if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
       iteration_ = next + L - 1;
       t_ = hS_->t;
+      harvest_events(iteration_);
+      maybe_mark(iteration_);
       continue;
     }
     const int stop = hS_->stop_iter;
+    harvest_events(stop - 1);
     iteration_ = stop;
     t_ = hS_->t;
     if (hS_->refac_mid && hS_->status == ST_RUNNING) {
@@ -671,9 +723,12 @@ bool Solver::phase2() {
     if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
       iteration_ = next + L - 1;
       t_ = hS_->t;
+      harvest_events(iteration_);
+      maybe_mark(iteration_);
       continue;
     }
     const int stop = hS_->stop_iter;
+    harvest_events(stop - 1);
     iteration_ = stop;
     t_ = hS_->t;
     if (hS_->status == ST_BFRT) throw Unsupported("bound flipping of boxed variables is not supported on the GPU path");
@@ -757,6 +812,7 @@ bool Solver::phase2() {
 int Solver::presolve() {
   if (!inited_) throw std::logic_error("initDualSimplex not called");
   const double t0 = now_ms();
+  t_start_ = t0;
   int st = 0;
   try {
     if (P_.perturb && checkPerturbNeed()) perturbCosts();
@@ -775,6 +831,9 @@ int Solver::presolve() {
     status_ = DS_ITERATION_LIMIT;
     message_ = "iteration limit";
     st = 2;
+    solve_ms_ += now_ms() - t0;
+    if (P_.time_kernels) nnzN_end_ = nnz_nonbasic();
+    return st;
   }
   solve_ms_ += now_ms() - t0;
   return st;
@@ -800,6 +859,7 @@ void Solver::solve() {
     Z_ = hS_->obj;
   }
   solve_ms_ += now_ms() - t0;
+  if (P_.time_kernels) nnzN_end_ = nnz_nonbasic();
 }
 
 void Solver::result(ds_result& r) const {
@@ -813,6 +873,12 @@ void Solver::result(ds_result& r) const {
   r.Z = Z_ - H_.offset;
   r.solve_ms = solve_ms_;
   r.refactor_ms = refactor_ms_;
+  r.mark_ms = mark_ms_;
+  r.spmv_ms = spmv_ms_;
+  r.spmv_launches = spmv_launches_;
+  r.kernel_k = k_;
+  r.nnz_nonbasic_mark = nnzN_mark_;
+  r.nnz_nonbasic_end = nnzN_end_;
 }
 
 void Solver::solution(double& Z, int& solved, std::string& message, std::vector<double>& x) {

@@ -128,6 +128,17 @@ class Solver {
   int refactors_ = 0, restarts_ = 0, phase1_iters_ = 0;
   double solve_ms_ = 0.0, refactor_ms_ = 0.0;
   uint64_t rng_state_ = 0;
+  // timing
+  double t_start_ = 0.0, mark_ms_ = 0.0, spmv_ms_ = 0.0;
+  int spmv_launches_ = 0;
+  bool marked_ = false;
+  double nnzN_mark_ = 0.0, nnzN_end_ = 0.0;
+  std::vector<hipEvent_t> ev_;
+  std::vector<int> ev_iter_;
+  int ev_used_ = 0;
+  void harvest_events(int last_done);
+  void maybe_mark(int done);
+  double nnz_nonbasic();
 
   DevModel M() const;
   DevIter I() const;
