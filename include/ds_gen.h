@@ -14,9 +14,10 @@
 extern "C" {
 #endif
 
-/* A(i,j) present with probability `density`, value U(0.1,1); x0 ~ U(0,1)^n;
- * b = A x0 + U(0,1)^m; c = -U(0,1)^n.  Rows are A x <= b.  Returns nnz, -2 if
- * `cap` is too small, -1 on bad arguments. */
+/* A(i,j) present with probability `density`, value U(0.1,1) (a column left
+ * empty gets one entry at a uniform random row, so the LP is bounded);
+ * x0 ~ U(0,1)^n; b = A x0 + U(0,1)^m; c = -U(0,1)^n.  Rows are A x <= b.
+ * Returns nnz, -2 if `cap` is too small, -1 on bad arguments. */
 int ds_gen_bernoulli_lp(int m, int n, double density, uint64_t seed, int* col_ptr, int* row_idx,
                         double* val, int cap, double* b, double* c);
 

@@ -29,6 +29,15 @@ extern "C" int ds_gen_bernoulli_lp(int m, int n, double density, uint64_t seed, 
         ++nnz;
       }
     }
+    if (nnz == col_ptr[j]) {
+      // An empty column with a negative cost would make the LP unbounded;
+      // give it one entry at a uniform random row so every instance is bounded.
+      if (nnz >= cap) return -2;
+      int r = (int)(rng.uniform() * m);
+      row_idx[nnz] = r < m ? r : m - 1;
+      val[nnz] = 0.1 + 0.9 * rng.uniform();
+      ++nnz;
+    }
     col_ptr[j + 1] = nnz;
   }
   std::vector<double> x0(n);

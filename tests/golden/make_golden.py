@@ -1,0 +1,126 @@
+"""Writes the golden fixtures under tests/golden/ (run in the build container, once).
+
+Sources:
+  * the reference's own known answers: the four LPs of tests/simple_tests_DS.cpp (with the
+    optima they state), the LU solve vectors of tests/test_math.cpp:12-49 and the
+    Matrix::dotUpdate cases exercised in tests/test_math_cuda.cpp:96-247, whose expected
+    outputs are derived by hand from src/cpu/matrix.cpp:261-346;
+  * scipy's HiGHS (an independent LP solver, not the reference) on seeded instances of the
+    benchmark workloads: objective, and the optimal basis when the optimum is nondegenerate;
+  * SHA-256 digests of generator outputs, pinning the seeded workload recipes.
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from parallel_ds_amd import lp as lpmod  # noqa: E402
+
+INF = float("inf")
+
+
+def analytic():
+    rows = [
+        dict(m=3, n=5, csr_val=[1, .5, .5, 1, 1, 1, -.5, .5], csr_ptr=[0, 3, 5, 8], csr_col=[1, 2, 4, 2, 3, 0, 2, 4],
+             costs=[2, -1, 3, -2, 1], lower_range=[1.5, 2, .5], upper_range=[1.5, 2, .5], lower_bound=[0] * 5,
+             range_type=[0, 0, 0], Z=-4.5, x=[0.5, 1.5, 0, 2, 0], source="tests/simple_tests_DS.cpp:12-44"),
+        dict(m=3, n=5, csr_val=[1, 1, -1, 5, 1, 3, -5, 1, 4], csr_ptr=[0, 3, 6, 9], csr_col=[0, 1, 4, 0, 2, 4, 0, 3, 4],
+             costs=[-14, 5, -2, 1, -8], lower_range=[5, 41, 15], upper_range=[5, 41, 15], lower_bound=[0] * 5,
+             range_type=[0, 0, 0], Z=-72.0, x=[7, 0, 0, 42, 2], source="tests/simple_tests_DS.cpp:47-79"),
+        dict(m=2, n=3, csr_val=[2, 1, 1, 3, 8, 2], csr_ptr=[0, 3, 6], csr_col=[0, 1, 2, 0, 1, 2], costs=[3, 2, 3],
+             lower_range=[None, 8], upper_range=[2, None], lower_bound=[0, 0, 1], range_type=[3, 4], Z=4.5,
+             x=[0, 0.75, 1], source="tests/simple_tests_DS.cpp:82-112"),
+        dict(m=3, n=5, csr_val=[1, 1, 1, 1, 3, 1, 1, 1], csr_ptr=[0, 3, 6, 8], csr_col=[0, 1, 2, 0, 1, 3, 0, 4],
+             costs=[-2, -3, 0, 0, 0], lower_range=[5, 9, 4], upper_range=[5, 9, 4], lower_bound=[0] * 5,
+             range_type=[0, 0, 0], Z=-12.0, x=[3, 2, 0, 0, 1], source="tests/simple_tests_DS.cpp:115-145"),
+    ]
+    # The reference's tests pass upper_bound = 0 for x >= 0 columns and 0 for unused range sides;
+    # an MPS read gives infinities there (problem.tpp:348-380), which is what is stored (None = inf).
+    return rows
+
+
+def dot_update_kat():
+    A = dict(csr_val=[1, .5, 4, 2, 1, 1, 1], csr_ptr=[0, 3, 5, 7], csr_col=[0, 1, 3, 0, 2, 1, 3], m=3, ncols=4)
+    rhs1 = [1.5, 2, .5]
+    rhs2 = [1.5, 2, .5, 1]
+    cols = [0, 1, 3]
+    return dict(A=A, cases=[
+        dict(method="UPDATE", vec1=rhs1, vec2=rhs1, alpha=-1, beta=1, set=0, cols=cols, sol_len=3,
+             expected=[-3, -1, -2]),
+        dict(method="UPDATE", vec1=rhs2, vec2=rhs1, alpha=-1, beta=1, set=1, cols=cols, sol_len=3,
+             expected=[-5, -1, -2.5]),
+        dict(method="UPDATE_T", vec1=rhs1, vec2=rhs2, alpha=1, beta=0, set=0, cols=cols, sol_len=3,
+             expected=[5.5, 1.25, 6.5]),
+        dict(method="UPDATE_T", vec1=rhs1, vec2=rhs2, alpha=-1, beta=1, set=1, cols=cols, sol_len=4,
+             expected=[-4, 0.75, 0, -5.5]),
+    ], source="tests/test_math_cuda.cpp:96-151 inputs; outputs derived from src/cpu/matrix.cpp:261-346")
+
+
+def lu_kat():
+    return dict(A=[[1, .5, 0], [2, 0, 1], [0, 1, 0]], b=[1.5, 2, .5], x=[1.25, 0.5, -0.5], y=[0.5, 0.5, 1.75],
+                source="tests/test_math.cpp:12-49 (A x = b and A^T y = b)")
+
+
+def highs(L):
+    A = sp.csc_matrix((L.val, L.row_idx, L.col_ptr), shape=(L.m, L.n))
+    eq = bool(np.all(L.range_type == 0))
+    t0 = time.time()
+    if eq:
+        r = linprog(L.costs, A_eq=A, b_eq=L.upper_range, bounds=(0, None), method="highs")
+    else:
+        r = linprog(L.costs, A_ub=A, b_ub=L.upper_range, bounds=(0, None), method="highs")
+    dt = time.time() - t0
+    assert r.status == 0, (L.name, r.message)
+    x = r.x
+    s = (L.upper_range - A @ x) if not eq else np.zeros(L.m)
+    basic = [int(j) for j in np.where(x > 1e-9)[0]] + [L.n + int(i) for i in np.where(s > 1e-9)[0]]
+    nondeg = (len(basic) == L.m) and not eq
+    out = dict(name=L.name, m=L.m, n=L.n, objective=float(r.fun), nondegenerate=bool(nondeg), highs_seconds=round(dt, 2))
+    if nondeg:
+        out["basis"] = sorted(basic)
+    return out
+
+
+def digest(L):
+    h = hashlib.sha256()
+    for a in (L.col_ptr, L.row_idx, L.val, L.costs, L.upper_range):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def main():
+    g = dict(analytic_lps=analytic(), dot_update=dot_update_kat(), lu=lu_kat())
+    with open(os.path.join(HERE, "known_answers.json"), "w") as f:
+        json.dump(g, f, indent=1)
+    d = dict(
+        config0_seed0=digest(lpmod.config0(0)),
+        kcol_m50_n100_k5_seed9=digest(lpmod.gen_kcol(50, 100, 5, 9)),
+        transport_10x20_seed2=digest(lpmod.gen_transport(10, 20, 2)),
+        config3_instance0=digest(lpmod.config3_instance(0)),
+        config1_seed1=digest(lpmod.config1()),
+    )
+    with open(os.path.join(HERE, "generator_digests.json"), "w") as f:
+        json.dump(d, f, indent=1)
+    out = dict(config0=[highs(lpmod.config0(s)) for s in range(20)])
+    out["transport"] = [highs(lpmod.gen_transport(S, D, s)) for (S, D, s) in [(10, 20, 2), (30, 60, 5), (50, 100, 2), (100, 200, 2)]]
+    out["config3"] = [highs(lpmod.config3_instance(k)) for k in range(32)]
+    out["kcol_m2000_n4000_k10_seed1"] = highs(lpmod.config1(seed=1, m=2000, n=4000, k=10))
+    out["kcol_m2000_n5000_k5_seed4"] = highs(lpmod.gen_kcol(2000, 5000, 5, 4))
+    out["transport_500x1000_seed2"] = highs(lpmod.config2())
+    for k in ("config0", "transport", "config3"):
+        for r in out[k]:
+            print(k, r["name"], r["objective"], r["nondegenerate"], flush=True)
+    with open(os.path.join(HERE, "highs_objectives.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,195 @@
+"""CPU tests that pin the oracle: the reference's known answers, closed forms, brute force,
+and an independent solver's objectives on the seeded workloads."""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import _oracle as ora
+import lpcheck
+from parallel_ds_amd import lp as lpmod
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KA = json.load(open(os.path.join(G, "known_answers.json")))
+HI = json.load(open(os.path.join(G, "highs_objectives.json")))
+INF = np.inf
+
+
+def lp_from_known(r):
+    m, n = r["m"], r["n"]
+    A = sp.csr_matrix((r["csr_val"], r["csr_col"], r["csr_ptr"]), shape=(m, n)).tocsc()
+    A.sort_indices()
+    lr = np.array([-INF if v is None else v for v in r["lower_range"]], float)
+    ur = np.array([INF if v is None else v for v in r["upper_range"]], float)
+    return lpmod.LP(m, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(float),
+                    np.array(r["costs"], float), np.array(r["lower_bound"], float), np.full(n, INF),
+                    np.full(n, 4, np.int32), lr, ur, np.array(r["range_type"], np.int32))
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_reference_analytic_lps(case):
+    r = KA["analytic_lps"][case]
+    L = lp_from_known(r)
+    res, x, basis = ora.solve(L)
+    assert res.status == 0 and res.solved == 1
+    assert abs(res.Z - r["Z"]) <= 1e-9 * max(1, abs(r["Z"]))
+    np.testing.assert_allclose(x[:L.n], r["x"], atol=1e-9)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_dot_update_known_answers(case):
+    A = KA["dot_update"]["A"]
+    c = KA["dot_update"]["cases"][case]
+    method = 0 if c["method"] == "UPDATE" else 1
+    sol = ora.dot_update((A["csr_val"], A["csr_ptr"], A["csr_col"], A["m"], A["ncols"]), c["vec1"], c["vec2"],
+                         c["alpha"], c["beta"], c["cols"], method, c["set"], c["sol_len"])
+    np.testing.assert_array_equal(sol, c["expected"])
+
+
+def test_lu_known_answer():
+    k = KA["lu"]
+    np.testing.assert_allclose(ora.dense_solve(k["A"], k["b"], False), k["x"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(ora.dense_solve(k["A"], k["b"], True), k["y"], rtol=0, atol=1e-15)
+
+
+def test_dense_lu_matches_numpy_with_pivoting():
+    rng = np.random.default_rng(3)
+    for k in (1, 5, 40):
+        A = rng.standard_normal((k, k))
+        if k > 1:
+            A[0, 0] = 0.0  # forces a row exchange
+        b = rng.standard_normal(k)
+        np.testing.assert_allclose(ora.dense_solve(A, b, False), np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(ora.dense_solve(A, b, True), np.linalg.solve(A.T, b), rtol=1e-10, atol=1e-12)
+
+
+def eta_matrix(eta, p):
+    E = np.eye(len(eta))
+    E[:, p] = eta
+    return E
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_pfi_matches_explicit_eta_product(transpose):
+    """PFIsolve (linalg.cpp:12-46) equals the product of the explicit eta matrices."""
+    rng = np.random.default_rng(5)
+    m, t = 7, 6
+    etas = rng.standard_normal((t, m))
+    piv = np.array([2, 5, 2, 0, 6, 5], np.int32)  # repeated positions included
+    b = rng.standard_normal(m)
+    M = np.eye(m)
+    for k in range(t):
+        M = eta_matrix(etas[k], piv[k]) @ M
+    ref = M.T @ b if transpose else M @ b
+    np.testing.assert_allclose(ora.pfi_solve(etas, piv, b, transpose), ref, rtol=1e-12, atol=1e-12)
+
+
+def brute_ratio(F, d, a):
+    """The sequential textbook rule of baseDualSimplex.tpp:546-572, written as a plain loop."""
+    q, theta = -1, 1e25
+    for i in F:
+        t = d[i] / a[i]
+        if t < theta:
+            q, theta = i, t
+        elif q >= 0 and abs(t - theta) < 1e-10 and abs(a[i]) > abs(a[q]):
+            q, theta = i, t
+    return q
+
+
+def test_ratio_test_rules():
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        n = 30
+        d = np.abs(rng.standard_normal(n)) * (rng.random(n) < 0.7)
+        a = rng.standard_normal(n)
+        if trial % 3 == 0:
+            d[:5] = 0.0  # degenerate ties
+        F = [i for i in range(n) if abs(a[i]) > 1e-5 and d[i] / a[i] >= 0]
+        q = ora.ratio_test(F, d, a, tie_rule=0)
+        assert q == brute_ratio(F, d, a)
+        # order-free rule: minimum ratio, then the largest |alpha| within 1e-10 of it
+        q2 = ora.ratio_test(F, d, a, tie_rule=1)
+        if F:
+            th = min(d[i] / a[i] for i in F)
+            near = [i for i in F if abs(d[i] / a[i] - th) < 1e-10]
+            assert q2 == max(near, key=lambda i: (abs(a[i]), -i))
+
+
+def vertex_enumeration(L):
+    """Minimum of c^T x over all basic feasible solutions of the computational form."""
+    AF = lpcheck.full_matrix(L)
+    l, u, c = lpcheck.comp_bounds(L)
+    best = None
+    N = L.n + L.m
+    for B in itertools.combinations(range(N), L.m):
+        Bm = AF[:, B]
+        if abs(np.linalg.det(Bm)) < 1e-12:
+            continue
+        x = np.zeros(N)
+        nb = [j for j in range(N) if j not in B]
+        for j in nb:
+            x[j] = l[j] if np.isfinite(l[j]) else 0.0
+        x[list(B)] = np.linalg.solve(Bm, -AF[:, nb] @ x[nb])
+        if np.all(x >= l - 1e-9) and np.all(x <= u + 1e-9):
+            z = c @ x
+            best = z if best is None else min(best, z)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_tiny_lps_against_vertex_enumeration(seed):
+    L = lpmod.gen_bernoulli(3, 4, 0.6, 100 + seed)
+    res, x, basis = ora.solve(L)
+    assert res.status == 0
+    z = vertex_enumeration(L)
+    assert abs(res.Z - z) <= 1e-9 * max(1, abs(z))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_config0_against_highs(seed):
+    L = lpmod.config0(seed)
+    g = HI["config0"][seed]
+    res, x, basis = ora.solve(L)
+    assert res.status == 0
+    assert abs(res.Z - g["objective"]) <= 1e-9 * abs(g["objective"])
+    pinf, dinf, _ = lpcheck.certificate(L, x, basis)
+    assert pinf <= 1e-7 and dinf <= 1e-7
+    if g["nondegenerate"]:
+        assert sorted(basis.tolist()) == g["basis"]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_transport_against_highs(case):
+    g = HI["transport"][case]
+    S, D, s = [(10, 20, 2), (30, 60, 5), (50, 100, 2)][case]
+    L = lpmod.gen_transport(S, D, s)
+    res, x, basis = ora.solve(L)
+    assert res.status == 0
+    assert abs(res.Z - g["objective"]) <= 1e-9 * abs(g["objective"])
+    A = sp.csc_matrix((L.val, L.row_idx, L.col_ptr), shape=(L.m, L.n))
+    assert np.abs(A @ x[:L.n] - L.upper_range).max() <= 1e-7
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_batched_family_instances_against_highs(k):
+    g = HI["config3"][k]
+    L = lpmod.config3_instance(k)
+    res, x, basis = ora.solve(L)
+    assert res.status == 0
+    assert abs(res.Z - g["objective"]) <= 1e-9 * abs(g["objective"])
+
+
+def test_iteration_cap_reports_limit():
+    L = lpmod.config0(0)
+    res, x, basis = ora.solve(L, max_iterations=10)
+    assert res.status == 3
+
+
+def test_perturbation_is_seeded_and_recorded():
+    L = lpmod.gen_transport(50, 100, 2)
+    r1, x1, _ = ora.solve(L, seed=1)
+    r2, x2, _ = ora.solve(L, seed=1)
+    assert r1.perturbed == 1 and r1.iterations == r2.iterations and r1.Z == r2.Z
