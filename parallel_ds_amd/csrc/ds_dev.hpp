@@ -38,7 +38,12 @@ struct DevIter {
   double* etas;      // eta_cap x m, dense eta columns
   int* eta_pos;      // eta_cap pivot positions
   double* E;         // eta_cap x eta_cap, E[k][j] = eta_k[eta_pos[j]]
-  double* chain;     // eta_cap, multipliers of the last FTRAN chain
+  double* MF;        // eta_cap x eta_cap, inverse of the unit triangular eta-pivot matrix
+  int* eta_last;     // eta_cap, previous eta index on the same pivot position (-1 if none)
+  double* x0;        // m, FTRAN result before the etas
+  int* cand_i;       // textbook CHUZC candidate lists, TB_LIST per workgroup
+  double* cand_th;
+  double* cand_a;
   int eta_cap;
   unsigned char* blocked;  // N
   int* blocked_list;       // MAX_BLOCKED

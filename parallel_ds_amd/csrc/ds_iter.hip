@@ -8,10 +8,11 @@
 #include "ds_kernels.hpp"
 #include "ds_reduce.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace pds {
 
 constexpr int BS = 256;
-constexpr int CHUNK = 32;  // eta rows staged in LDS per step of the PFI chains
 
 __device__ inline bool stopped(const DevScalars* S) { return S->status != ST_RUNNING || S->refac_mid != 0; }
 
@@ -197,75 +198,62 @@ __global__ __launch_bounds__(BS) void k_dualcheck(DevModel M, DevIter I, int it)
 // ---------------------------------------------------------------------------
 // BTRAN for a unit vector: rho = B0^{-T} E_1^T ... E_t^T e_p
 // (sequentialDualSimplex.cpp:89-92, linalg.cpp:34-44).  The transposed etas
-// touch only the pivot positions, so the chain runs on a (t+1)-entry sparse
-// vector in one workgroup; the kernel-inverse gather then runs on all rows.
+// only write the pivot positions; their values u solve a unit triangular
+// system whose inverse MF is maintained by the basis update, so u = MF^T b is
+// one parallel matvec (b_k = eta_k[p] while position p still holds its unit
+// entry).  The kernel-inverse gather then runs on all rows in k_btran_out.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_btran_chain(DevModel M, DevIter I, DevFactor F, int t, int p_fixed) {
+__global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed) {
   DevScalars* S = I.S;
   if (p_fixed < 0 && stopped(S)) return;
   const int p = p_fixed >= 0 ? p_fixed : S->p_idx;
   const int m = M.m;
+  const int cap = I.eta_cap;
   __shared__ int pos[ETA_MAX + 1];
   __shared__ int slot[ETA_MAX + 1];
   __shared__ double v[ETA_MAX + 1];
-  __shared__ double Ech[CHUNK][ETA_MAX + 1];
+  __shared__ double bv[ETA_MAX];
+  __shared__ double u[ETA_MAX];
   __shared__ int cnt[ETA_MAX + 1];
   __shared__ int off[ETA_MAX + 2];
+  __shared__ int kmax_p, kmin_p;
   const int ns = t + 1;
+  if (threadIdx.x == 0) {
+    kmax_p = -1;
+    kmin_p = INT_MAX;
+  }
   for (int s = threadIdx.x; s < ns; s += BS) pos[s] = (s == 0) ? p : I.eta_pos[s - 1];
   __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS)
+    if (pos[1 + k] == p) {
+      atomicMax(&kmax_p, k);
+      atomicMin(&kmin_p, k);
+    }
   for (int s = threadIdx.x; s < ns; s += BS) {
     int f = s;
     for (int s2 = 0; s2 < s; ++s2)
       if (pos[s2] == pos[s]) { f = s2; break; }
     slot[s] = f;
-    v[s] = 0.0;
   }
   __syncthreads();
-  if (threadIdx.x == 0) v[slot[0]] = 1.0;
+  for (int k = threadIdx.x; k < t; k += BS) bv[k] = (k >= kmax_p) ? I.etas[(size_t)k * m + p] : 0.0;
   __syncthreads();
-  // Wave 0 keeps v in registers: lane l owns slots l, l+64, ...
-  constexpr int RPL = (ETA_MAX + 1 + WAVE - 1) / WAVE;
-  double vr[RPL];
-  const int lane = threadIdx.x % WAVE;
-  const bool w0 = threadIdx.x < WAVE;
-  if (w0)
-    for (int r = 0; r < RPL; ++r) {
-      int s = lane + WAVE * r;
-      vr[r] = (s < ns) ? v[s] : 0.0;
-    }
-  for (int khi = t - 1; khi >= 0; khi -= CHUNK) {
-    int klo = khi - CHUNK + 1;
-    if (klo < 0) klo = 0;
-    __syncthreads();
-    for (int e = threadIdx.x; e < (khi - klo + 1) * ns; e += BS) {
-      int kk = e / ns, s = e % ns;
-      int k = klo + kk;
-      Ech[kk][s] = (s == 0) ? I.etas[(size_t)k * m + p] : I.E[(size_t)k * I.eta_cap + (s - 1)];
-    }
-    __syncthreads();
-    if (w0) {
-      for (int k = khi; k >= klo; --k) {
-        double acc = 0.0;
-#pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-          int s = lane + WAVE * r;
-          if (s < ns && slot[s] == s) acc += Ech[k - klo][s] * vr[r];
-        }
-        acc = wave_sum(acc);
-        const int tgt = slot[1 + k];
-#pragma unroll
-        for (int r = 0; r < RPL; ++r)
-          if (tgt == lane + WAVE * r) vr[r] = acc;
-      }
-    }
+  for (int k = threadIdx.x; k < t; k += BS) {
+    double acc = 0.0;
+    for (int k2 = k; k2 < t; ++k2) acc += I.MF[(size_t)k2 * cap + k] * bv[k2];
+    u[k] = acc;
   }
   __syncthreads();
-  if (w0)
-    for (int r = 0; r < RPL; ++r) {
-      int s = lane + WAVE * r;
-      if (s < ns) v[s] = vr[r];
+  // Final values at the support: each position keeps the last assignment of
+  // the reverse sweep, i.e. that of its smallest eta index.
+  for (int s = threadIdx.x; s < ns; s += BS) {
+    double val = 0.0;
+    if (slot[s] == s) {
+      if (s == 0) val = (kmax_p < 0) ? 1.0 : u[kmin_p];
+      else val = u[s - 1];
     }
+    v[s] = val;
+  }
   __syncthreads();
   // Right-hand side of the kernel solve: rhs_P = v_P - C^T y_S, with
   // y_S = v at slack positions (ordered by slot, prefix-summed offsets).
@@ -430,6 +418,139 @@ __device__ inline bool ratio_candidate(const DevModel& M, const DevIter& I, int 
   return (dj >= 0 && a > EPS_A) || (dj <= 0 && a < -EPS_A);
 }
 
+// Shared tail of CHUZC once the entering position fi is known.
+__device__ inline void chuzc_finish(const DevModel& M, const DevIter& I, int phase, int it, int fi) {
+  DevScalars* S = I.S;
+  if (fi == INT_MAX) {
+    raise_status(S, ST_NO_RATIO, it);
+    return;
+  }
+  const int q = I.nonbasis[fi];
+  const double arq = I.alpha_r[fi];
+  S->q_idx = fi;
+  S->q = q;
+  S->alpha_rq = arq;
+  S->theta = I.d[q] / arq;
+  if (phase == 2) {
+    S->unbound_checked = 0;
+    if (M.bt[q] == BT_BOXED) {
+      double span = M.u[q] - M.l[q];
+      if (fabs(S->delta) - span * fabs(arq) > 0) raise_status(S, ST_BFRT, it);
+    }
+  } else {
+    int8_t tq = M.bt[q];
+    double dq = I.d[q];
+    double corr = 0.0;
+    if ((tq == BT_UPPER || tq == BT_FREE) && dq > EPS_D) corr = 1.0;
+    else if ((tq == BT_LOWER || tq == BT_FREE) && dq < -EPS_D) corr = -1.0;
+    S->corr = corr;
+    if (S->any_blocked) {
+      for (int b = 0; b < S->nblocked; ++b) I.blocked[I.blocked_list[b]] = 0;
+      S->nblocked = 0;
+      S->any_blocked = 0;
+    }
+  }
+}
+
+// Textbook ratio test in one launch.  Each workgroup keeps the candidates
+// within EPS_Z of its own minimum ratio (every candidate within EPS_Z of the
+// global minimum is among them); the last workgroup merges the lists.  A
+// workgroup whose list overflowed and may hold global candidates makes the
+// last workgroup rescan all positions.
+constexpr int TB_LIST = 8;
+__global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int neg = S->neg;
+  const int nN = M.N - M.m;
+  double mn = BIG;
+  int nF = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    double a = I.alpha_r[i];
+    if (neg) a = -a;
+    int j = I.nonbasis[i];
+    if (!ratio_candidate(M, I, phase, j, a)) continue;
+    ++nF;
+    mn = fmin(mn, I.d[j] / a);
+  }
+  mn = block_min<BS>(mn);
+  nF = block_sumi<BS>(nF);
+  __shared__ int lc;
+  __shared__ int li[TB_LIST];
+  __shared__ double lt[TB_LIST], la[TB_LIST];
+  if (threadIdx.x == 0) lc = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    double a = I.alpha_r[i];
+    if (neg) a = -a;
+    int j = I.nonbasis[i];
+    if (!ratio_candidate(M, I, phase, j, a)) continue;
+    double th = I.d[j] / a;
+    if (th < mn + EPS_Z) {
+      int slot = atomicAdd(&lc, 1);
+      if (slot < TB_LIST) {
+        li[slot] = i;
+        lt[slot] = th;
+        la[slot] = fabs(a);
+      }
+    }
+  }
+  __syncthreads();
+  const int b = blockIdx.x;
+  if (threadIdx.x == 0) {
+    I.part_d[b] = mn;
+    I.part_i[2 * b] = nF;
+    I.part_i[2 * b + 1] = lc;
+  }
+  for (int e = threadIdx.x; e < TB_LIST; e += BS)
+    if (e < lc) {
+      I.cand_i[b * TB_LIST + e] = li[e];
+      I.cand_th[b * TB_LIST + e] = lt[e];
+      I.cand_a[b * TB_LIST + e] = la[e];
+    }
+  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  double g = BIG;
+  int tot = 0;
+  for (int w = threadIdx.x; w < (int)gridDim.x; w += BS) {
+    g = fmin(g, I.part_d[w]);
+    tot += I.part_i[2 * w];
+  }
+  g = block_min<BS>(g);
+  tot = block_sumi<BS>(tot);
+  int rescan = 0;
+  double best = -1.0;
+  int bi = INT_MAX;
+  for (int w = threadIdx.x; w < (int)gridDim.x; w += BS) {
+    int n = I.part_i[2 * w + 1];
+    if (n > TB_LIST && I.part_d[w] < g + EPS_Z) rescan = 1;
+    n = n < TB_LIST ? n : TB_LIST;
+    for (int e = 0; e < n; ++e)
+      if (fabs(I.cand_th[w * TB_LIST + e] - g) < EPS_Z) argmax_merge(best, bi, I.cand_a[w * TB_LIST + e], I.cand_i[w * TB_LIST + e]);
+  }
+  rescan = block_sumi<BS>(rescan);
+  if (rescan) {
+    best = -1.0;
+    bi = INT_MAX;
+    for (int i = threadIdx.x; i < nN; i += BS) {
+      double a = I.alpha_r[i];
+      if (neg) a = -a;
+      int j = I.nonbasis[i];
+      if (!ratio_candidate(M, I, phase, j, a)) continue;
+      if (fabs(I.d[j] / a - g) < EPS_Z) argmax_merge(best, bi, fabs(a), i);
+    }
+  }
+  block_argmax<BS>(best, bi);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_CHUZC1] = 0;
+  S->nF = tot;
+  S->tmp0 = g;
+  if (tot == 0) {
+    raise_status(S, ST_NO_RATIO, it);
+    return;
+  }
+  chuzc_finish(M, I, phase, it, bi);
+}
+
 __global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase, int harris, int it) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
@@ -499,35 +620,7 @@ __global__ __launch_bounds__(BS) void k_chuzc2(DevModel M, DevIter I, int phase,
   block_argmax<BS>(fb, fi);
   if (threadIdx.x != 0) return;
   S->ctr[CTR_CHUZC2] = 0;
-  if (fi == INT_MAX) {
-    raise_status(S, ST_NO_RATIO, it);
-    return;
-  }
-  const int q = I.nonbasis[fi];
-  const double arq = I.alpha_r[fi];
-  S->q_idx = fi;
-  S->q = q;
-  S->alpha_rq = arq;
-  S->theta = I.d[q] / arq;
-  if (phase == 2) {
-    S->unbound_checked = 0;
-    if (M.bt[q] == BT_BOXED) {
-      double span = M.u[q] - M.l[q];
-      if (fabs(S->delta) - span * fabs(arq) > 0) raise_status(S, ST_BFRT, it);
-    }
-  } else {
-    int8_t tq = M.bt[q];
-    double dq = I.d[q];
-    double corr = 0.0;
-    if ((tq == BT_UPPER || tq == BT_FREE) && dq > EPS_D) corr = 1.0;
-    else if ((tq == BT_LOWER || tq == BT_FREE) && dq < -EPS_D) corr = -1.0;
-    S->corr = corr;
-    if (S->any_blocked) {
-      for (int b = 0; b < S->nblocked; ++b) I.blocked[I.blocked_list[b]] = 0;
-      S->nblocked = 0;
-      S->any_blocked = 0;
-    }
-  }
+  chuzc_finish(M, I, phase, it, fi);
 }
 
 // ---------------------------------------------------------------------------
@@ -579,69 +672,30 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   out[i] = x;
 }
 
-// Forward PFI chain: multipliers s_k = value at eta_pos[k] just before eta k
-// is applied (linalg.cpp:21-32), from x0 at the pivot positions and the
-// eta values there (E).  One workgroup; the update of all positions follows
-// in k_ftran_out.
-__global__ __launch_bounds__(BS) void k_ftran_chain(DevModel M, DevIter I, const double* x0, int t, int check) {
-  DevScalars* S = I.S;
-  if (check && stopped(S)) return;
-  __shared__ int pk[ETA_MAX];
-  __shared__ int last[ETA_MAX];
-  __shared__ double xp[ETA_MAX];
-  __shared__ double Ec[ETA_MAX][CHUNK + 1];
-  __shared__ double s_sh[ETA_MAX];
-  for (int k = threadIdx.x; k < t; k += BS) {
-    pk[k] = I.eta_pos[k];
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS) {
-    int lst = -1;
-    for (int k2 = k - 1; k2 >= 0; --k2)
-      if (pk[k2] == pk[k]) { lst = k2; break; }
-    last[k] = lst;
-    xp[k] = x0[pk[k]];
-  }
-  const int lane = threadIdx.x % WAVE;
-  const bool w0 = threadIdx.x < WAVE;
-  for (int k0 = 0; k0 < t; k0 += CHUNK) {
-    int k1 = min(t, k0 + CHUNK);
-    __syncthreads();
-    // Ec[j][kk] = E[j][k0+kk] = eta_j[p_{k0+kk}] for j < k1
-    for (int e = threadIdx.x; e < k1 * (k1 - k0); e += BS) {
-      int jj = e / (k1 - k0), kk = e % (k1 - k0);
-      Ec[jj][kk] = I.E[(size_t)jj * I.eta_cap + (k0 + kk)];
-    }
-    __syncthreads();
-    if (w0) {
-      for (int k = k0; k < k1; ++k) {
-        int lst = last[k];
-        double acc = 0.0;
-        for (int jj = lst + 1 + lane; jj < k; jj += WAVE) acc += s_sh[jj] * Ec[jj][k - k0];
-        acc = wave_sum(acc);
-        double base = (lst < 0) ? xp[k] : s_sh[lst] * Ec[lst][k - k0];
-        if (lane == 0) s_sh[k] = base + acc;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      }
-    }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS) I.chain[k] = s_sh[k];
-}
-
-// x = x0 with the forward etas applied, using the chain multipliers.
+// Forward etas (linalg.cpp:21-32) on x0 = B0^{-1} a.  The multiplier of eta
+// k is the value at its pivot just before it is applied, s = MF (mask .* x0
+// at the pivots); every workgroup forms s from MF, then updates its slice:
+// x = x0 + sum_k s_k eta_k, with a pivot position overwritten by s_k eta_k[p_k].
 // mode 0: out[i] = x; mode 1: x_var[basis[i]] = x (dualSimplexInit's scatter).
-__global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* x0, double* out, int t,
-                                                  int mode, int check) {
+__global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* __restrict__ x0,
+                                                  double* out, int t, int mode, int check) {
   DevScalars* S = I.S;
   if (check && stopped(S)) return;
+  __shared__ double xm[ETA_MAX];
   __shared__ double sk[ETA_MAX];
   __shared__ int pk[ETA_MAX];
+  const int cap = I.eta_cap;
   for (int k = threadIdx.x; k < t; k += BS) {
-    sk[k] = I.chain[k];
-    pk[k] = I.eta_pos[k];
+    int q = I.eta_pos[k];
+    pk[k] = q;
+    xm[k] = (I.eta_last[k] < 0) ? x0[q] : 0.0;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS) {
+    const double* row = I.MF + (size_t)k * cap;
+    double acc = 0.0;
+    for (int j = 0; j <= k; ++j) acc += row[j] * xm[j];
+    sk[k] = acc;
   }
   __syncthreads();
   const int i = blockIdx.x * BS + threadIdx.x;
@@ -665,15 +719,19 @@ __global__ __launch_bounds__(BS) void k_gather(const double* src, const int* map
 }
 
 __global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int rows, int cols,
-                                             const double* __restrict__ xin, double* __restrict__ dst,
-                                             const int* omap, const DevScalars* S, int check) {
+                                             const double* __restrict__ xin, const int* imap,
+                                             double* __restrict__ dst, const int* omap, const DevScalars* S,
+                                             int check) {
   if (check && stopped(S)) return;
   const int w = (blockIdx.x * BS + threadIdx.x) / WAVE;
   const int lane = threadIdx.x % WAVE;
   if (w >= rows) return;
   const double* row = A + (size_t)w * cols;
   double acc = 0.0;
-  for (int jj = lane; jj < cols; jj += WAVE) acc += row[jj] * xin[jj];
+  if (imap)
+    for (int jj = lane; jj < cols; jj += WAVE) acc += row[jj] * xin[imap[jj]];
+  else
+    for (int jj = lane; jj < cols; jj += WAVE) acc += row[jj] * xin[jj];
   acc = wave_sum(acc);
   if (lane == 0) dst[omap ? omap[w] : w] = acc;
 }
@@ -776,14 +834,32 @@ __global__ __launch_bounds__(BS) void k_update(DevModel M, DevIter I, int phase,
     }
   }
   if (!last_block(&S->ctr[CTR_UPD])) return;
-  // New E entries: row t (eta_t at the older pivots) and column t.
+  // New E entries: row t (eta_t at the older pivots) and column t (older etas
+  // at the new pivot); then row t of MF = (I - L)^{-1}, where L[t][i] =
+  // E[i][t] for i at or after the previous eta on the same position.
   const int cap = I.eta_cap;
+  __shared__ double ecol[ETA_MAX];
+  __shared__ int s_last;
+  if (threadIdx.x == 0) s_last = -1;
+  __syncthreads();
   for (int jj = threadIdx.x; jj < t; jj += BS) {
     I.E[(size_t)t * cap + jj] = eta[I.eta_pos[jj]];
-    I.E[(size_t)jj * cap + t] = I.etas[(size_t)jj * M.m + p_idx];
+    double e = I.etas[(size_t)jj * M.m + p_idx];
+    I.E[(size_t)jj * cap + t] = e;
+    ecol[jj] = e;
+    if (I.eta_pos[jj] == p_idx) atomicMax(&s_last, jj);
+  }
+  __syncthreads();
+  const int lastt = s_last;
+  for (int j = threadIdx.x; j < t; j += BS) {
+    double acc = 0.0;
+    for (int i = (j > lastt ? j : lastt); i < t; ++i) acc += ecol[i] * I.MF[(size_t)i * cap + j];
+    I.MF[(size_t)t * cap + j] = acc;
   }
   if (threadIdx.x != 0) return;
   S->ctr[CTR_UPD] = 0;
+  I.MF[(size_t)t * cap + t] = 1.0;
+  I.eta_last[t] = lastt;
   I.E[(size_t)t * cap + t] = eta[p_idx];
   I.eta_pos[t] = p_idx;
   S->t = t + 1;
@@ -1009,36 +1085,43 @@ void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it
 }
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed) {
-  k_btran_chain<<<1, BS, 0, s>>>(M, I, F, t, p_fixed);
+  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, p_fixed);
   k_btran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0);
 }
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
-                 const double* cvec, double* out, int by_col, const DevScalars* S, int check) {
-  k_spmv<<<nblk(M.N, WAVE), WAVE, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
+                 const double* cvec, double* out, int by_col, const DevScalars* S, int check, hipEvent_t ev0,
+                 hipEvent_t ev1) {
+  if (ev0 && ev1)
+    hipExtLaunchKernelGGL(k_spmv, dim3(nblk(M.N, WAVE)), dim3(WAVE), 0, s, ev0, ev1, 0, M, posN, v, alpha, beta,
+                          cvec, out, by_col, S, check);
+  else
+    k_spmv<<<nblk(M.N, WAVE), WAVE, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
 }
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);
+  if (!harris) {
+    k_chuzc_tb<<<g, BS, 0, s>>>(M, I, phase, it);
+    return;
+  }
   k_chuzc1<<<g, BS, 0, s>>>(M, I, phase, harris, it);
   k_chuzc2<<<g, BS, 0, s>>>(M, I, phase, harris, it);
 }
 void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
                       double* out, int check) {
-  k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, out, check);
   if (t > 0) {
-    k_ftran_chain<<<1, BS, 0, s>>>(M, I, out, t, check);
-    k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, out, out, t, 0, check);
+    k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, I.x0, check);
+    k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, I.x0, out, t, 0, check);
+  } else {
+    k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, out, check);
   }
 }
 void launch_ftran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const double* b,
                         double* out, int t, int mode, int check) {
   // out (by position) = B^{-1} b; mode 1 scatters the result into x via basis.
-  double* tmp = (mode == 0) ? out : I.w2;
-  if (F.k > 0) {
-    k_gather<<<nblk(F.k, BS), BS, 0, s>>>(b, F.row_R, I.wN, F.k, I.S, check);
-    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.Kinv, F.k, F.k, I.wN, tmp, F.pos_P, I.S, check);
-  }
+  double* tmp = (mode == 0 && t == 0) ? out : I.x0;
+  if (F.k > 0)
+    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.Kinv, F.k, F.k, b, F.row_R, tmp, F.pos_P, I.S, check);
   k_ftran_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, b, tmp, I.S, check);
-  if (t > 0) k_ftran_chain<<<1, BS, 0, s>>>(M, I, tmp, t, check);
   if (t > 0 || mode == 1) k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, tmp, out, t, mode, check);
 }
 void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, double* v,
@@ -1047,7 +1130,7 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
   if (t > 0) k_btran_dense_eta<<<1, 1024, 0, s>>>(M, I, v, t);
   if (F.k > 0) {
     k_btran_dense_rhs<<<nblk(F.k, BS), BS, 0, s>>>(F, v, I.wN);
-    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.KinvT, F.k, F.k, I.wN, y, F.row_R, I.S, 0);
+    k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.KinvT, F.k, F.k, I.wN, nullptr, y, F.row_R, I.S, 0);
   }
   k_btran_dense_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, v, y);
 }

@@ -11,7 +11,8 @@ void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed);
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
-                 const double* cvec, double* out, int by_col, const DevScalars* S, int check);
+                 const double* cvec, double* out, int by_col, const DevScalars* S, int check,
+                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it);
 void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
                       double* out, int check);
@@ -39,10 +40,16 @@ struct GJWork {
   int* piv;       // k
   int* perm;      // k
   int* flag;      // singular-pivot flag
+  double* Pinv;   // NB x NB inverse of the pivot block
+  int* rowsA;     // tournament candidate lists (ceil(k/256)+1) x NB, two buffers
+  int* rowsB;
+  int* nA;
+  int* nB;
   int use_mfma;   // trailing update on FP64 MFMA (1) or plain FMA (0)
 };
 constexpr int GJ_NB = 64;
 void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, const DevFactor& F, double* W);
-void launch_gj_invert(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT);
+void launch_gj_panels(hipStream_t s, int k, const GJWork& G);
+void launch_gj_unscramble(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT);
 
 }  // namespace pds

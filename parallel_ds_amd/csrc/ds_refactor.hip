@@ -1,10 +1,11 @@
 // Refactorization on gfx950: explicit inverse of the basis kernel K by
 // blocked Gauss-Jordan elimination with partial pivoting.
 //
-// Per panel of NB columns: LU of the panel (one workgroup, partial
-// pivoting) -> row swaps across the matrix -> the panel is saved and its
-// columns reset to identity columns -> W = P_mid^{-1} * (mid rows) by two
-// small triangular solves -> every other row gets the rank-NB update
+// Per panel of NB columns: pivot rows chosen by tournament pivoting (LDS LU
+// of row chunks, merged in rounds) -> row swaps across the matrix -> the
+// panel is saved and its columns reset to identity columns -> W = P_mid^{-1}
+// * (mid rows) with the explicit inverse of the pivot block -> every other
+// row gets the rank-NB update
 // rows -= P_saved * W, an FP64 MFMA GEMM (v_mfma_f64_16x16x4f64).  After the
 // last panel the column permutation is undone while Kinv and its transpose
 // are written.  Total work 2k^3 flops, all but O(k^2 NB) in the MFMA update.
@@ -29,50 +30,177 @@ __global__ __launch_bounds__(256) void k_build_kernel(DevModel M, const int* bas
   }
 }
 
-// LU with partial pivoting of the panel W[j:k, j:j+nb] into Pf (ld NB).
-__global__ __launch_bounds__(1024) void k_panel_lu(const double* W, int k, int j, int nb, double* Pf, int* piv,
-                                                   int* flag) {
-  const int R = k - j;
-  for (int e = threadIdx.x; e < R * nb; e += 1024) {
-    int r = e / nb, cc = e % nb;
-    Pf[(size_t)r * NB + cc] = W[(size_t)(j + r) * k + j + cc];
+// Pivot rows of a panel by tournament: each workgroup factors up to TS_ROWS
+// rows of the panel W[., j:j+nb] (their current values) in LDS with partial
+// pivoting and keeps its nb pivot rows; rounds of merges (up to four lists
+// per workgroup) reduce the lists to one.  Level 0 reads rows j..k-1
+// directly (in_rows == nullptr).  The final workgroup also writes the LU of
+// the winners' block (Pf), its inverse (Pinv) and the row exchanges
+// piv[j..j+nb) that bring the winners to rows j..j+nb-1 in pivot order.
+constexpr int TS_ROWS = 256;
+constexpr int TS_THREADS = 1024;
+
+// Pivot rows of a panel by tournament: each workgroup factors up to TS_ROWS
+// rows of the panel W[., j:j+nb] (their current values) in LDS with partial
+// pivoting and keeps its nb pivot rows; rounds of merges (up to four lists
+// per workgroup) reduce the lists to one.  Level 0 reads rows j..k-1
+// directly (in_rows == nullptr).  The final workgroup also writes the LU of
+// the winners' block (Pf), its inverse (Pinv) and the row exchanges
+// piv[j..j+nb) that bring the winners to rows j..j+nb-1 in pivot order.
+__global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int j, int nb, const int* in_rows,
+                                                     const int* in_n, int in_groups, int* out_rows, int* out_n,
+                                                     int final_, double* Pf, double* Pinv, int* piv, int* flag) {
+  __shared__ double A[TS_ROWS][NB + 1];
+  __shared__ int rid[TS_ROWS];
+  __shared__ int goff[5];
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int n = 0;
+    goff[0] = 0;
+    for (int g = 0; g < 4; ++g) {
+      int gi = 4 * w + g;
+      n += (in_rows && gi < in_groups) ? in_n[gi] : 0;
+      goff[g + 1] = n;
+    }
   }
   __syncthreads();
-  __shared__ int s_piv;
-  for (int s = 0; s < nb; ++s) {
+  int R;
+  if (!in_rows) {
+    const int r0 = j + w * TS_ROWS;
+    R = min(k, r0 + TS_ROWS) - r0;
+    for (int t = tid; t < R; t += TS_THREADS) rid[t] = r0 + t;
+  } else {
+    R = goff[4];
+    for (int t = tid; t < R; t += TS_THREADS) {
+      int g = 0;
+      while (t >= goff[g + 1]) ++g;
+      rid[t] = in_rows[(4 * w + g) * NB + (t - goff[g])];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < R * nb; e += TS_THREADS) {
+    int r = e / nb, c = e % nb;
+    A[r][c] = W[(size_t)rid[r] * k + j + c];
+  }
+  __syncthreads();
+  const int steps = min(nb, R);
+  for (int st = 0; st < steps; ++st) {
     double bv = -1.0;
     int bi = INT_MAX;
-    for (int r = s + threadIdx.x; r < R; r += 1024) argmax_merge(bv, bi, fabs(Pf[(size_t)r * NB + s]), r);
-    block_argmax<1024>(bv, bi);
-    if (threadIdx.x == 0) {
-      // No finite nonzero pivot (zero or NaN column): keep the row, flag the
-      // factor as singular; every index stays inside the panel.
-      if (bi == INT_MAX || !(bv > 0.0)) {
-        if (bi == INT_MAX || bi < s || bi >= R) bi = s;
-        *flag = 1;
+    for (int r = st + tid; r < R; r += TS_THREADS) {
+      double a = fabs(A[r][st]);
+      if (a >= 0.0) argmax_merge(bv, bi, a, r);
+    }
+    block_argmax<TS_THREADS>(bv, bi);
+    if (bi == INT_MAX || !(bv > 0.0)) {
+      // No nonzero pivot in this column among these rows (common for a
+      // sparse kernel): leave the column, keep row st in place.  Only the
+      // final round makes this an error.
+      if (final_ && tid == 0) *flag = 1;
+      __syncthreads();
+      continue;
+    }
+    if (bi != st) {
+      for (int c = tid; c < nb; c += TS_THREADS) {
+        double x = A[st][c];
+        A[st][c] = A[bi][c];
+        A[bi][c] = x;
       }
-      s_piv = bi;
-      piv[j + s] = j + bi;
+      if (tid == 0) {
+        int x = rid[st];
+        rid[st] = rid[bi];
+        rid[bi] = x;
+      }
     }
     __syncthreads();
-    const int pr = s_piv;
-    if (pr != s)
-      for (int cc = threadIdx.x; cc < nb; cc += 1024) {
-        double a = Pf[(size_t)s * NB + cc];
-        Pf[(size_t)s * NB + cc] = Pf[(size_t)pr * NB + cc];
-        Pf[(size_t)pr * NB + cc] = a;
+    const double dv = A[st][st];
+    for (int r = st + 1 + tid; r < R; r += TS_THREADS) A[r][st] = A[r][st] / dv;
+    __syncthreads();
+    const int wdt = nb - st - 1;
+    if (wdt > 0)
+      for (int e = tid; e < (R - st - 1) * wdt; e += TS_THREADS) {
+        int r = st + 1 + e / wdt, c = st + 1 + e % wdt;
+        A[r][c] = A[r][c] - A[r][st] * A[st][c];
       }
     __syncthreads();
-    const double dv = Pf[(size_t)s * NB + s];
-    for (int r = s + 1 + threadIdx.x; r < R; r += 1024) Pf[(size_t)r * NB + s] = Pf[(size_t)r * NB + s] / dv;
+  }
+  for (int t = tid; t < steps; t += TS_THREADS) out_rows[w * NB + t] = rid[t];
+  if (tid == 0) out_n[w] = steps;
+  if (!final_) return;
+  if (steps < nb && tid == 0) *flag = 1;
+  // Final round: LU factors, inverse of the winners' block, row exchanges.
+  for (int e = tid; e < nb * nb; e += TS_THREADS) Pf[(size_t)(e / nb) * NB + (e % nb)] = A[e / nb][e % nb];
+  __syncthreads();
+  // Gauss-Jordan on [B | I], B = the winners' original block in pivot order
+  // (same pivots as the LU above, so no further exchanges).
+  double(*G)[NB + 1] = &A[NB];       // rows NB..2NB-1: B
+  double(*H)[NB + 1] = &A[2 * NB];   // rows 2NB..3NB-1: I -> B^{-1}
+  __shared__ double prow[2 * NB];
+  __shared__ double fcol[NB];
+  for (int e = tid; e < nb * nb; e += TS_THREADS) {
+    int r = e / nb, c = e % nb;
+    G[r][c] = W[(size_t)rid[r] * k + j + c];
+    H[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int t = 0; t < nb; ++t) {
+    const double pv = G[t][t];
+    for (int c = tid; c < 2 * nb; c += TS_THREADS) prow[c] = (c < nb) ? G[t][c] : H[t][c - nb];
+    for (int r = tid; r < nb; r += TS_THREADS) fcol[r] = G[r][t] / pv;
     __syncthreads();
-    const int w = nb - s - 1;
-    if (w > 0)
-      for (int e = threadIdx.x; e < (R - s - 1) * w; e += 1024) {
-        int r = s + 1 + e / w, cc = s + 1 + e % w;
-        Pf[(size_t)r * NB + cc] = Pf[(size_t)r * NB + cc] - Pf[(size_t)r * NB + s] * Pf[(size_t)s * NB + cc];
+    for (int e = tid; e < nb * 2 * nb; e += TS_THREADS) {
+      int r = e / (2 * nb), c = e % (2 * nb);
+      double x = (c < nb) ? G[r][c] : H[r][c - nb];
+      x = (r == t) ? x / pv : x - fcol[r] * prow[c];
+      if (c < nb) G[r][c] = x;
+      else H[r][c - nb] = x;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * nb; e += TS_THREADS) Pinv[(size_t)(e / nb) * NB + (e % nb)] = H[e / nb][e % nb];
+  // Sequential exchanges: position j+s receives winner rid[s].  Touched
+  // positions live in a small LDS map (position -> row now there), searched
+  // by wave 0.
+  __shared__ int key[2 * NB], val[2 * NB];
+  __shared__ int nk;
+  if (tid == 0) nk = 0;
+  __syncthreads();
+  if (tid < WAVE) {
+    const int lane = tid;
+    for (int s2 = 0; s2 < nb; ++s2) {
+      const int target = rid[s2];
+      const int pj = j + s2;
+      const int n = nk;
+      int fv = -1, fj = -1, fl = -1;
+      for (int q = lane; q < n; q += WAVE) {
+        if (val[q] == target) fv = q;
+        if (key[q] == pj) fj = q;
       }
-    __syncthreads();
+      for (int o = 32; o > 0; o >>= 1) {
+        fv = max(fv, __shfl_xor(fv, o, WAVE));
+        fj = max(fj, __shfl_xor(fj, o, WAVE));
+      }
+      const int loc = (fv >= 0) ? key[fv] : target;
+      for (int q = lane; q < n; q += WAVE)
+        if (key[q] == loc) fl = q;
+      for (int o = 32; o > 0; o >>= 1) fl = max(fl, __shfl_xor(fl, o, WAVE));
+      if (lane == 0) {
+        const int cj = (fj >= 0) ? val[fj] : pj;
+        piv[pj] = loc;
+        int kj = fj, kl = fl, cnt = n;
+        if (kj < 0) { key[cnt] = pj; kj = cnt++; }
+        val[kj] = target;
+        if (loc != pj) {
+          if (kl < 0) { key[cnt] = loc; kl = cnt++; }
+          val[kl] = cj;
+        }
+        nk = cnt;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
   }
 }
 
@@ -100,28 +228,36 @@ __global__ __launch_bounds__(256) void k_panel_save(double* W, int k, int j, int
   *w = (r == j + cc) ? 1.0 : 0.0;
 }
 
-// Mid rows: X <- U^{-1} L^{-1} X for every column (one lane per column).
-__global__ __launch_bounds__(64) void k_mid_solve(double* W, int k, int j, int nb, const double* Pf) {
-  __shared__ double LU[NB][NB + 1];
+// Mid rows: X <- Pinv X for every column, Pinv = inverse of the pivot block.
+__global__ __launch_bounds__(256) void k_mid_gemm(double* W, int k, int j, int nb, const double* Pinv) {
+  __shared__ double Pi[NB][NB + 1];
   __shared__ double X[NB][64 + 1];
-  const int lane = threadIdx.x;
-  for (int e = lane; e < nb * nb; e += 64) LU[e / nb][e % nb] = Pf[(size_t)(e / nb) * NB + (e % nb)];
-  const int c = blockIdx.x * 64 + lane;
-  const bool live = c < k;
-  for (int s = 0; s < nb; ++s) X[s][lane] = live ? W[(size_t)(j + s) * k + c] : 0.0;
+  const int c0 = blockIdx.x * 64;
+  for (int e = threadIdx.x; e < NB * NB; e += 256) {
+    int r = e / NB, t = e % NB;
+    Pi[r][t] = (r < nb && t < nb) ? Pinv[(size_t)r * NB + t] : 0.0;
+  }
+  for (int e = threadIdx.x; e < NB * 64; e += 256) {
+    int r = e / 64, cc = e % 64;
+    X[r][cc] = (r < nb && c0 + cc < k) ? W[(size_t)(j + r) * k + c0 + cc] : 0.0;
+  }
   __syncthreads();
-  for (int s = 1; s < nb; ++s) {
-    double a = X[s][lane];
-    for (int t = 0; t < s; ++t) a = a - LU[s][t] * X[t][lane];
-    X[s][lane] = a;
+  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+  double acc[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int t = 0; t < nb; ++t) {
+    double pv[4], xv[4];
+    for (int a = 0; a < 4; ++a) pv[a] = Pi[tr + 16 * a][t];
+    for (int b = 0; b < 4; ++b) xv[b] = X[t][tc + 16 * b];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(pv[a], xv[b], acc[a][b]);
   }
-  for (int s = nb - 1; s >= 0; --s) {
-    double a = X[s][lane];
-    for (int t = s + 1; t < nb; ++t) a = a - LU[s][t] * X[t][lane];
-    X[s][lane] = a / LU[s][s];
-  }
-  if (live)
-    for (int s = 0; s < nb; ++s) W[(size_t)(j + s) * k + c] = X[s][lane];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      int r = tr + 16 * a, c = c0 + tc + 16 * b;
+      if (r < nb && c < k) W[(size_t)(j + r) * k + c] = acc[a][b];
+    }
 }
 
 // C[r][c] -= sum_t P[r][t] * B[t][c] for rows outside [skip0, skip1).
@@ -203,18 +339,6 @@ __global__ __launch_bounds__(256) void k_gemm_sub_fma(double* C, int ldc, int ro
     }
 }
 
-// perm = composition of the column swaps applied in reverse order.
-__global__ void k_perm(const int* piv, int* perm, int k) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int c = 0; c < k; ++c) perm[c] = c;
-  for (int s = k - 1; s >= 0; --s) {
-    int r = piv[s];
-    int a = perm[s];
-    perm[s] = perm[r];
-    perm[r] = a;
-  }
-}
-
 // Kinv[r][c] = W[r][perm[c]] and KinvT[c][r] = the same value (tiled transpose).
 __global__ __launch_bounds__(256) void k_unscramble(const double* W, int k, const int* perm, double* Kinv,
                                                     double* KinvT) {
@@ -243,21 +367,42 @@ void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, con
   k_build_kernel<<<(F.k + 255) / 256, 256, 0, s>>>(M, basis, F, W);
 }
 
-void launch_gj_invert(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT) {
+void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
   if (k <= 0) return;
   for (int j = 0; j < k; j += NB) {
     int nb = (k - j < NB) ? (k - j) : NB;
-    k_panel_lu<<<1, 1024, 0, s>>>(G.W, k, j, nb, G.Pf, G.piv, G.flag);
+    int groups = (k - j + TS_ROWS - 1) / TS_ROWS;
+    int* in_rows = G.rowsA;
+    int* in_n = G.nA;
+    k_tslu<<<groups, TS_THREADS, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, groups == 1 ? 1 : 0, G.Pf,
+                                   G.Pinv, G.piv, G.flag);
+    int* out_rows = G.rowsB;
+    int* out_n = G.nB;
+    while (groups > 1) {
+      int g2 = (groups + 3) / 4;
+      k_tslu<<<g2, TS_THREADS, 0, s>>>(G.W, k, j, nb, in_rows, in_n, groups, out_rows, out_n, g2 == 1 ? 1 : 0, G.Pf,
+                                 G.Pinv, G.piv, G.flag);
+      int* tr = in_rows;
+      in_rows = out_rows;
+      out_rows = tr;
+      int* tn = in_n;
+      in_n = out_n;
+      out_n = tn;
+      groups = g2;
+    }
     k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv);
     k_panel_save<<<(int)(((long)k * nb + 255) / 256), 256, 0, s>>>(G.W, k, j, nb, G.Pb);
-    k_mid_solve<<<(k + 63) / 64, 64, 0, s>>>(G.W, k, j, nb, G.Pf);
+    k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv);
     dim3 grid((k + 63) / 64, (k + 63) / 64);
     if (G.use_mfma)
       k_gemm_sub<<<grid, 256, 0, s>>>(G.W, k, k, k, G.Pb, G.W + (size_t)j * k, k, nb, j, j + nb);
     else
       k_gemm_sub_fma<<<grid, 256, 0, s>>>(G.W, k, k, k, G.Pb, G.W + (size_t)j * k, k, nb, j, j + nb);
   }
-  k_perm<<<1, 1, 0, s>>>(G.piv, G.perm, k);
+}
+
+void launch_gj_unscramble(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT) {
+  if (k <= 0) return;
   dim3 g2((k + 31) / 32, (k + 31) / 32);
   k_unscramble<<<g2, 256, 0, s>>>(G.W, k, G.perm, Kinv, KinvT);
 }

@@ -103,15 +103,17 @@ class Solver {
   DBuf<double> cval_, rval_, c_, l_, u_;
   DBuf<int8_t> bt_;
   // iterate
-  DBuf<double> x_, d_, beta_, f_, rho_, alpha_r_, alpha_q_, tau_, w1_, w2_, wN_, etas_, E_, chain_, part_d_,
-      list_val_, ylist_val_;
-  DBuf<int> basis_, nonbasis_, posN_, eta_pos_, blocked_list_, list_idx_, ylist_row_, list_n_, part_i_;
+  DBuf<double> x_, d_, beta_, f_, rho_, alpha_r_, alpha_q_, tau_, w1_, w2_, wN_, etas_, E_, MF_, x0_, part_d_,
+      list_val_, ylist_val_, cand_th_, cand_a_;
+  DBuf<int> basis_, nonbasis_, posN_, eta_pos_, eta_last_, blocked_list_, list_idx_, ylist_row_, list_n_, part_i_,
+      cand_i_;
   DBuf<unsigned char> blocked_;
   DBuf<DevScalars> S_;
   DevScalars* hS_ = nullptr;  // pinned mirror
   // factor
-  DBuf<double> Kinv_, KinvT_, GW_, Pf_, Pb_, ccv_, ccv2_;
-  DBuf<int> pos_P_, row_R_, Ridx_, Pidx_, slackpos_, slackrow_, crp_, cci_, ccp_, ccr_, piv_, perm_, flag_;
+  DBuf<double> Kinv_, KinvT_, GW_, Pf_, Pb_, Pinv_, ccv_, ccv2_;
+  DBuf<int> pos_P_, row_R_, Ridx_, Pidx_, slackpos_, slackrow_, crp_, cci_, ccp_, ccr_, piv_, perm_, flag_, rowsA_,
+      rowsB_, nA_, nB_;
   int k_ = 0;
   int eta_cap_ = 0;
   int gj_mfma_ = 1;  // trailing update of the inversion on FP64 MFMA

@@ -124,3 +124,28 @@ def test_transport_vs_oracle(gpu, S, D, seed):
     assert res.status == r_o.status == 0
     assert res.perturbed == r_o.perturbed
     assert abs(sol.Z - r_o.Z) <= 1e-9 * abs(r_o.Z)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_btran_ftran_large_kernel(gpu, seed):
+    """A basis whose kernel exceeds one tournament chunk (k > 256): multi-round pivoting."""
+    L = lpmod.gen_bernoulli(600, 1200, 0.02, 40 + seed)
+    r_o, x_o, b_o = ora.solve(L)
+    assert r_o.status == 0
+    AF = lpcheck.full_matrix(L)
+    B = AF[:, b_o]
+    assert (b_o < L.n).sum() > 256
+    with DualSimplex(L, CAP) as s:
+        s.initDualSimplex()
+        s.set_basis(b_o)
+        rng = np.random.default_rng(seed)
+        b = rng.standard_normal(L.m)
+        x = s.ftran(b)
+        ref = np.linalg.solve(B, b)
+        assert np.abs(x - ref).max() <= 1e-9 * max(1, np.abs(ref).max())
+        for p in (3, 301, L.m - 2):
+            rho = s.btran(p)
+            e = np.zeros(L.m)
+            e[p] = 1
+            ref = np.linalg.solve(B.T, e)
+            assert np.abs(rho - ref).max() <= 1e-9 * max(1, np.abs(ref).max())
