@@ -129,6 +129,37 @@ int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis /* n */);
  * launch in *ms and the algorithmic bytes of one launch in *bytes. */
 int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, double* bytes);
 
+/* ---- Batched small LPs: one persistent launch, one workgroup per LP ----
+ * No counterpart in the reference (it solves one LP per process,
+ * src/main.cpp:55-92); each LP follows the same algorithm and rules as
+ * ds_solve.  LPs are packing LPs  min c^T x, A x <= b, x >= 0.  Arrays are
+ * concatenated over the batch: col_ptr holds n_k+1 entries per LP (each
+ * starting at 0), row_idx/val hold nnz_off[k]..nnz_off[k+1]-1, b holds m_k
+ * entries and c n_k entries per LP. */
+typedef struct {
+  int refactor_freq;      /* reinversion interval (default 150) */
+  int max_iterations;     /* per LP, 0 = no cap */
+  int perturb;
+  uint64_t perturb_seed;
+  int block;              /* workgroup size: 128, 256 or 512 */
+  int lds_inv_max_m;      /* LPs with m <= this keep B^{-1} in LDS */
+  int wgs_per_cu;         /* resident workgroups per CU of the persistent grid */
+  int use_graph;          /* capture the launch in a hipGraph */
+} ds_batch_params;
+
+void ds_batch_default_params(ds_batch_params* p);
+int ds_batch_load(ds_ctx* ctx, int count, const int* m, const int* n, const long long* nnz_off, const int* col_ptr,
+                  const int* row_idx, const double* val, const double* b, const double* c);
+/* Starting bases (m_k variable ids per LP, concatenated); NULL restores the slack basis. */
+int ds_batch_set_warm(ds_ctx* ctx, const int* basis);
+int ds_batch_solve(ds_ctx* ctx, const ds_batch_params* p, double* ms);
+int ds_batch_replay(ds_ctx* ctx, double* ms);  /* relaunch the captured graph */
+int ds_batch_results(ds_ctx* ctx, int* status, double* obj, int* iters, int* basis /* sum m */,
+                     double* x /* sum (n+m) */);
+/* Autotuner: times workgroup size x LDS threshold x occupancy on a sample of
+ * `sample` LPs; writes the fastest into *best (its other fields are kept). */
+int ds_batch_tune(ds_ctx* ctx, int sample, ds_batch_params* best, double* best_ms, int* tried);
+
 /* Trailing-update tile check: C(64x64, row-major) -= P(:, :nb) * B(nb x 64), where
  * P is 64x64 row-major (columns >= nb ignored) and B is nb x 64 row-major; the
  * FP64 MFMA kernel when use_mfma, else the plain FMA kernel. */

@@ -29,6 +29,12 @@ class Result(C.Structure):
                 ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("perturb", C.c_int),
+                ("perturb_seed", C.c_uint64), ("block", C.c_int), ("lds_inv_max_m", C.c_int),
+                ("wgs_per_cu", C.c_int), ("use_graph", C.c_int)]
+
+
 class DsError(RuntimeError):
     def __init__(self, code, where, detail=""):
         super().__init__("%s failed: %s %s" % (where, ERRORS.get(code, code), detail))
@@ -66,6 +72,13 @@ _SIGS = {
     "ds_pivot_row": (I32, [VP, VP, VP]),
     "ds_time_pivot_row": (I32, [VP, VP, I32, C.POINTER(F64), C.POINTER(F64)]),
     "ds_debug_gemm_tile": (I32, [VP, VP, VP, I32, I32]),
+    "ds_batch_default_params": (None, [C.POINTER(BatchParams)]),
+    "ds_batch_load": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP]),
+    "ds_batch_set_warm": (I32, [VP, VP]),
+    "ds_batch_solve": (I32, [VP, C.POINTER(BatchParams), C.POINTER(F64)]),
+    "ds_batch_replay": (I32, [VP, C.POINTER(F64)]),
+    "ds_batch_results": (I32, [VP, VP, VP, VP, VP, VP]),
+    "ds_batch_tune": (I32, [VP, I32, C.POINTER(BatchParams), C.POINTER(F64), C.POINTER(I32)]),
     "ds_gen_bernoulli_lp": (I32, [I32, I32, F64, U64, VP, VP, VP, I32, VP, VP]),
     "ds_gen_kcol_lp": (I32, [I32, I32, I32, U64, VP, VP, VP, VP, VP]),
     "ds_gen_transport": (I32, [I32, I32, U64, VP, VP, VP, VP, VP]),

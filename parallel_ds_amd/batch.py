@@ -1,0 +1,103 @@
+"""Batched small-LP solves over the C-ABI (one persistent launch, one workgroup per LP)."""
+import ctypes as C
+
+import numpy as np
+
+from ._native import BatchParams, DsError, lib, ptr
+
+
+def default_batch_params(**kw):
+    p = BatchParams()
+    lib().ds_batch_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class BatchSolver:
+    """Packing LPs (min c^T x, A x <= b, x >= 0) solved together on the GPU."""
+
+    def __init__(self, lps, device=0):
+        L = lib()
+        self._ctx = C.c_void_p()
+        rc = L.ds_create(device, C.byref(self._ctx))
+        if rc != 0:
+            raise DsError(rc, "ds_create", "(no usable GPU?)")
+        self.lps = list(lps)
+        self.count = len(self.lps)
+        m = np.array([lp.m for lp in self.lps], np.int32)
+        n = np.array([lp.n for lp in self.lps], np.int32)
+        nnz = np.array([lp.nnz for lp in self.lps], np.int64)
+        nnz_off = np.zeros(self.count + 1, np.int64)
+        nnz_off[1:] = np.cumsum(nnz)
+        col_ptr = np.concatenate([lp.col_ptr for lp in self.lps]).astype(np.int32)
+        row_idx = np.concatenate([lp.row_idx for lp in self.lps]).astype(np.int32)
+        val = np.concatenate([lp.val for lp in self.lps])
+        b = np.concatenate([lp.upper_range for lp in self.lps])
+        c = np.concatenate([lp.costs for lp in self.lps])
+        self.m, self.n = m, n
+        self.moff = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+        self.voff = np.concatenate([[0], np.cumsum(m + n)]).astype(np.int64)
+        self._check(L.ds_batch_load(self._ctx, self.count, ptr(m), ptr(n), ptr(nnz_off), ptr(col_ptr),
+                                    ptr(row_idx), ptr(val), ptr(b), ptr(c)), "ds_batch_load")
+
+    def _check(self, rc, where):
+        if rc != 0:
+            detail = lib().ds_last_error(self._ctx)
+            raise DsError(rc, where, detail.decode() if detail else "")
+
+    def close(self):
+        if self._ctx:
+            lib().ds_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_warm(self, bases):
+        """bases: list of m_k-length variable-id arrays, or None for the slack basis."""
+        if bases is None:
+            self._check(lib().ds_batch_set_warm(self._ctx, None), "ds_batch_set_warm")
+            return
+        flat = np.concatenate([np.asarray(b, np.int32) for b in bases]).astype(np.int32)
+        self._check(lib().ds_batch_set_warm(self._ctx, ptr(flat)), "ds_batch_set_warm")
+
+    def solve(self, params=None):
+        p = params if params is not None else default_batch_params()
+        ms = C.c_double()
+        self._check(lib().ds_batch_solve(self._ctx, C.byref(p), C.byref(ms)), "ds_batch_solve")
+        return ms.value
+
+    def replay(self):
+        ms = C.c_double()
+        self._check(lib().ds_batch_replay(self._ctx, C.byref(ms)), "ds_batch_replay")
+        return ms.value
+
+    def results(self, with_x=False):
+        st = np.zeros(self.count, np.int32)
+        obj = np.zeros(self.count)
+        it = np.zeros(self.count, np.int32)
+        basis = np.zeros(int(self.moff[-1]), np.int32)
+        x = np.zeros(int(self.voff[-1])) if with_x else None
+        self._check(lib().ds_batch_results(self._ctx, ptr(st), ptr(obj), ptr(it), ptr(basis), ptr(x)),
+                    "ds_batch_results")
+        bases = [basis[self.moff[k]:self.moff[k + 1]] for k in range(self.count)]
+        xs = [x[self.voff[k]:self.voff[k + 1]] for k in range(self.count)] if with_x else None
+        return st, obj, it, bases, xs
+
+    def tune(self, sample=256, base=None):
+        best = base if base is not None else default_batch_params()
+        ms = C.c_double()
+        tried = C.c_int()
+        self._check(lib().ds_batch_tune(self._ctx, sample, C.byref(best), C.byref(ms), C.byref(tried)),
+                    "ds_batch_tune")
+        return best, ms.value, tried.value

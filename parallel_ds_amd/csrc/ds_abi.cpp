@@ -4,10 +4,13 @@
 #include <string>
 
 #include "ds_abi.h"
+#include "ds_batch_host.hpp"
 #include "ds_solver.hpp"
 
 struct ds_ctx {
   pds::Solver* solver = nullptr;
+  pds::BatchSolver* batch = nullptr;
+  int device = 0;
   std::string err;
 };
 
@@ -44,6 +47,14 @@ int guard(ds_ctx* ctx, F&& f) {
   }
 }
 
+template <typename F>
+int bguard(ds_ctx* ctx, F&& f) {
+  if (!ctx) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver&) {
+    if (!ctx->batch) ctx->batch = new pds::BatchSolver(ctx->device);
+    f(*ctx->batch);
+  });
+}
 }  // namespace
 
 extern "C" {
@@ -70,6 +81,7 @@ int ds_create(int device, ds_ctx** out) {
   ds_ctx* c = new ds_ctx();
   try {
     c->solver = new pds::Solver(device);
+    c->device = device;
   } catch (const std::exception& e) {
     delete c;
     return DS_EHIP;
@@ -80,6 +92,7 @@ int ds_create(int device, ds_ctx** out) {
 
 int ds_destroy(ds_ctx* ctx) {
   if (!ctx) return DS_EINVAL;
+  delete ctx->batch;
   delete ctx->solver;
   delete ctx;
   return DS_OK;
@@ -218,6 +231,60 @@ int ds_pivot_row(ds_ctx* ctx, const double* rho, double* alpha) {
 int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, double* bytes) {
   if (!rho || !ms || !bytes || reps <= 0) return DS_EINVAL;
   return guard(ctx, [&](pds::Solver& s) { s.time_pivot_row(rho, reps, *ms, *bytes); });
+}
+
+void ds_batch_default_params(ds_batch_params* p) {
+  if (!p) return;
+  p->refactor_freq = 150;
+  p->max_iterations = 200000;
+  p->perturb = 1;
+  p->perturb_seed = 0x5eedULL;
+  p->block = 256;
+  p->lds_inv_max_m = 64;
+  p->wgs_per_cu = 2;
+  p->use_graph = 1;
+}
+
+int ds_batch_load(ds_ctx* ctx, int count, const int* m, const int* n, const long long* nnz_off, const int* col_ptr,
+                  const int* row_idx, const double* val, const double* b, const double* c) {
+  if (count <= 0 || !m || !n || !nnz_off || !col_ptr || !row_idx || !val || !b || !c) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.load(count, m, n, nnz_off, col_ptr, row_idx, val, b, c); });
+}
+
+int ds_batch_set_warm(ds_ctx* ctx, const int* basis) {
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.set_warm(basis); });
+}
+
+int ds_batch_solve(ds_ctx* ctx, const ds_batch_params* p, double* ms) {
+  if (!p) return DS_EINVAL;
+  if (p->block != 128 && p->block != 256 && p->block != 512) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    if (B.count() == 0) throw std::logic_error("no batch loaded");
+    double t = B.solve(*p);
+    if (ms) *ms = t;
+  });
+}
+
+int ds_batch_replay(ds_ctx* ctx, double* ms) {
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    double t = B.replay();
+    if (ms) *ms = t;
+  });
+}
+
+int ds_batch_results(ds_ctx* ctx, int* status, double* obj, int* iters, int* basis, double* x) {
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.results(status, obj, iters, basis, x); });
+}
+
+int ds_batch_tune(ds_ctx* ctx, int sample, ds_batch_params* best, double* best_ms, int* tried) {
+  if (!best || sample <= 0) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    double ms = 0;
+    int t = 0;
+    B.tune(sample, *best, ms, t);
+    if (best_ms) *best_ms = ms;
+    if (tried) *tried = t;
+  });
 }
 
 int ds_debug_gemm_tile(double* C, const double* P, const double* B, int nb, int use_mfma) {

@@ -1,0 +1,828 @@
+// Batched dual simplex on gfx950: thousands of small LPs in one persistent
+// launch, one workgroup per LP at a time, instances taken from an atomic
+// work queue until it is exhausted.
+//
+// Each workgroup runs the reference's algorithm (phase 1 = minimize the sum
+// of dual infeasibilities, baseDualSimplex.tpp:799-934; phase 2 with dual
+// steepest-edge pricing and the textbook ratio test,
+// sequentialDualSimplex.cpp:253-515; the same restart, perturbation and
+// refactor rules) on an explicit dense basis inverse: rows of B^{-1} give
+// BTRAN, columns give FTRAN, and each basis change is the product-form row
+// update B^{-1} <- E B^{-1}.  "Refactor" recomputes B^{-1} by Gauss-Jordan
+// with partial pivoting.  All vectors of the LP live in LDS; the inverse
+// lives in LDS for the small size class and in the workgroup's slice of a
+// global scratch buffer (L2/MALL-resident) otherwise.
+#include "ds_batch.hpp"
+#include "ds_reduce.hpp"
+#include "ds_rules.hpp"
+
+namespace pds {
+
+namespace {
+
+__device__ inline double dev_uniform(unsigned long long& s) {
+  unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct BShared {
+  int status;
+  int iteration;
+  int perturbed;
+  int cycle;
+  int p, p_idx, q, q_idx;
+  int neg;
+  int nF;
+  int dinf;
+  int sing;
+  int nblocked;
+  int blocked[64];
+  double delta, theta, obj, f_p, corr, tmp;
+  unsigned long long rng;
+};
+
+constexpr int ST_OK = -1;  // internal: keep iterating
+
+template <int BS>
+struct LpSolve {
+  int m, n, N;
+  const int* cp;
+  const int* ri;
+  const double* cv;
+  const int* rp;
+  const int* ci;
+  const double* rv;
+  const double* corig;
+  const double* l;
+  const double* u;
+  const int8_t* bt;
+  double* c;
+  double* x;
+  double* d;
+  double* wN;
+  double* beta;
+  double* f;
+  double* rho;
+  double* aq;
+  double* tau;
+  double* w;
+  double* prow;
+  double* ar;
+  double* sortbuf;
+  int* basis;
+  int* nonbasis;
+  int* posN;
+  int* piv;
+  double* Binv;
+  BShared* sh;
+  int K, max_iter, perturb_on;
+  unsigned long long seed;
+  int tid;
+
+  __device__ void sync() { __syncthreads(); }
+
+  // out[i] = sum_r Binv[i][r] * v[r], one wavefront per row (coalesced).
+  __device__ void rowdot(const double* v, double* out) {
+    const int lane = tid % WAVE, wv = tid / WAVE;
+    for (int i = wv; i < m; i += BS / WAVE) {
+      double s = 0.0;
+      for (int r = lane; r < m; r += WAVE) s += Binv[i * m + r] * v[r];
+      s = wave_sum(s);
+      if (lane == 0) out[i] = s;
+    }
+    sync();
+  }
+
+  // Column j of [A | I] dotted with v, in CSC order (matrix.cpp:273-287 with alpha).
+  __device__ double col_dot(int j, const double* v, double alpha) const {
+    double acc = 0.0;
+    if (j < n) {
+      for (int e = cp[j]; e < cp[j + 1]; ++e) acc = acc + cv[e] * v[ri[e]] * alpha;
+    } else {
+      acc = acc + 1.0 * v[j - n] * alpha;
+    }
+    return acc;
+  }
+
+  // B^{-1} from the current basis: Gauss-Jordan with partial pivoting.
+  __device__ bool reinvert() {
+    for (int i = 0; i < m; ++i)
+      for (int cc = tid; cc < m; cc += BS) Binv[i * m + cc] = 0.0;
+    sync();
+    for (int i = tid; i < m; i += BS) {
+      int j = basis[i];
+      if (j < n) {
+        for (int e = cp[j]; e < cp[j + 1]; ++e) Binv[ri[e] * m + i] = cv[e];
+      } else {
+        Binv[(j - n) * m + i] = 1.0;
+      }
+    }
+    sync();
+    for (int s = 0; s < m; ++s) {
+      double bv = -1.0;
+      int bi = INT_MAX;
+      for (int r = s + tid; r < m; r += BS) {
+        double a = fabs(Binv[r * m + s]);
+        if (a >= 0.0) argmax_merge(bv, bi, a, r);
+      }
+      block_argmax<BS>(bv, bi);
+      if (bi == INT_MAX || !(bv > 0.0)) {
+        if (tid == 0) sh->sing = 1;
+        sync();
+        return false;
+      }
+      if (tid == 0) piv[s] = bi;
+      if (bi != s)
+        for (int cc = tid; cc < m; cc += BS) {
+          double a = Binv[s * m + cc];
+          Binv[s * m + cc] = Binv[bi * m + cc];
+          Binv[bi * m + cc] = a;
+        }
+      sync();
+      const double pv = Binv[s * m + s];
+      sync();
+      for (int cc = tid; cc < m; cc += BS) Binv[s * m + cc] = ((cc == s) ? 1.0 : Binv[s * m + cc]) / pv;
+      for (int i = tid; i < m; i += BS) w[i] = (i == s) ? 0.0 : Binv[i * m + s];
+      sync();
+      for (int i = 0; i < m; ++i) {
+        if (i == s) continue;
+        const double wi = w[i];
+        for (int cc = tid; cc < m; cc += BS) {
+          double a = (cc == s) ? 0.0 : Binv[i * m + cc];
+          Binv[i * m + cc] = a - wi * Binv[s * m + cc];
+        }
+      }
+      sync();
+    }
+    for (int s = m - 1; s >= 0; --s) {
+      int r = piv[s];
+      if (r != s)
+        for (int i = tid; i < m; i += BS) {
+          double a = Binv[i * m + s];
+          Binv[i * m + s] = Binv[i * m + r];
+          Binv[i * m + r] = a;
+        }
+      sync();
+    }
+    return true;
+  }
+
+  // initBetaWeights (tpp:34-70): identity or exact row norms of B^{-1}.
+  __device__ void init_beta(bool identity) {
+    if (identity) {
+      for (int i = tid; i < m; i += BS) beta[i] = 1.0;
+    } else {
+      const int lane = tid % WAVE, wv = tid / WAVE;
+      for (int i = wv; i < m; i += BS / WAVE) {
+        double s2 = 0.0;
+        for (int r = lane; r < m; r += WAVE) s2 += Binv[i * m + r] * Binv[i * m + r];
+        s2 = wave_sum(s2);
+        if (lane == 0) beta[i] = s2;
+      }
+    }
+    sync();
+  }
+
+  // y = B^{-T} c_B ; d_N = c_N - A_N^T y  (sequentialDualSimplex.cpp:119-132)
+  __device__ void init_reduced_costs() {
+    for (int r = tid; r < m; r += BS) {
+      double y = 0.0;
+      for (int i = 0; i < m; ++i) y += c[basis[i]] * Binv[i * m + r];
+      w[r] = y;
+    }
+    sync();
+    for (int i = tid; i < n; i += BS) {
+      int j = nonbasis[i];
+      d[j] = col_dot(j, w, -1.0) + c[j] * 1.0;
+    }
+    sync();
+  }
+
+  // setPrimalVars (tpp:74-101)
+  __device__ void set_primal() {
+    for (int i = tid; i < n; i += BS) {
+      int j = nonbasis[i];
+      double v = 0.0;
+      switch (bt[j]) {
+        case BT_FIXED: v = l[j]; break;
+        case BT_FREE: v = 0.0; break;
+        case BT_BOXED: v = (d[j] > 0) ? l[j] : u[j]; break;
+        case BT_UPPER: v = u[j]; break;
+        case BT_LOWER: v = l[j]; break;
+      }
+      x[j] = v;
+    }
+    sync();
+  }
+
+  // dualSimplexInit (sequentialDualSimplex.cpp:230-250)
+  __device__ void dual_init() {
+    set_primal();
+    for (int r = tid; r < m; r += BS) {
+      double s = 0.0 * 1.0;
+      for (int e = rp[r]; e < rp[r + 1]; ++e) {
+        int col = ci[e];
+        double bf = (posN[col] >= 0) ? x[col] : 0.0;
+        s = s + rv[e] * bf * -1.0;
+      }
+      double bf = (posN[n + r] >= 0) ? x[n + r] : 0.0;
+      s = s + 1.0 * bf * -1.0;
+      w[r] = s;
+    }
+    sync();
+    rowdot(w, tau);
+    for (int i = tid; i < m; i += BS) x[basis[i]] = tau[i];
+    sync();
+    init_reduced_costs();
+    double s = 0.0;
+    for (int j = tid; j < N; j += BS) s += c[j] * x[j];
+    s = block_sum<BS>(s);
+    if (tid == 0) sh->obj = s;
+    sync();
+  }
+
+  // minimizeDualInfeasibilityInit + initPhase1PricingVector
+  __device__ void phase1_init() {
+    init_reduced_costs();
+    double z = 0.0;
+    int cnt = 0;
+    for (int j = tid; j < N; j += BS) {
+      double wt = 0.0;
+      if (posN[j] >= 0) {
+        int8_t t = bt[j];
+        double dj = d[j];
+        if ((t == BT_UPPER || t == BT_FREE) && dj > EPS_D) wt = -1.0;
+        else if ((t == BT_LOWER || t == BT_FREE) && dj < -EPS_D) wt = 1.0;
+        z += wt * dj;
+        cnt += dual_infeas(t, dj);
+      }
+      wN[j] = wt;
+    }
+    z = block_sum<BS>(z);
+    cnt = block_sumi<BS>(cnt);
+    if (tid == 0) {
+      sh->obj = z;
+      sh->dinf = cnt;
+    }
+    sync();
+    for (int r = tid; r < m; r += BS) {
+      double s = 0.0;
+      for (int e = rp[r]; e < rp[r + 1]; ++e) s += rv[e] * wN[ci[e]];
+      s += wN[n + r];
+      w[r] = s;
+    }
+    sync();
+    rowdot(w, f);
+  }
+
+  // checkPerturbNeed (tpp:223-234): bitonic sort of the costs, then the
+  // tolerance-unique count of the sorted sequence.
+  __device__ bool perturb_need() {
+    int P2 = 1;
+    while (P2 < N) P2 <<= 1;
+    for (int j = tid; j < P2; j += BS) sortbuf[j] = (j < N) ? c[j] : 1e300;
+    sync();
+    for (int k = 2; k <= P2; k <<= 1)
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int i = tid; i < P2; i += BS) {
+          int ixj = i ^ jj;
+          if (ixj > i) {
+            bool up = (i & k) == 0;
+            double a = sortbuf[i], b = sortbuf[ixj];
+            if ((a > b) == up) {
+              sortbuf[i] = b;
+              sortbuf[ixj] = a;
+            }
+          }
+        }
+        sync();
+      }
+    if (tid == 0) {
+      int uniq = 0;
+      double last = 0.0;
+      for (int i = 0; i < N; ++i)
+        if (i == 0 || !(fabs(last - sortbuf[i]) < EPS_COSTS)) {
+          last = sortbuf[i];
+          ++uniq;
+        }
+      sh->tmp = (uniq < PERTURB_RATIO * (double)N) ? 1.0 : 0.0;
+    }
+    sync();
+    return sh->tmp != 0.0;
+  }
+
+  // perturbCosts (tpp:238-275) with the seeded stream.
+  __device__ void perturb_costs() {
+    double s = 0.0;
+    for (int j = tid; j < N; j += BS) s += c[j];
+    s = block_sum<BS>(s);
+    if (tid == 0) {
+      const double wts[10] = {0.01, 0.1, 1, 2, 5, 10, 20, 30, 40, 100};
+      const double mean = s / (double)N;
+      const double min_p = fmin(1e-2 * EPS_D, PSI);
+      const double max_p = fmax(1e+3 * EPS_D, PSI * 10 * mean);
+      unsigned long long st = sh->rng;
+      for (int i = 0; i < N; ++i) {
+        double magnitude = 100 * EPS_D + PSI * c[i];
+        double pert;
+        if (bt[i] != BT_LOWER && bt[i] != BT_FREE) pert = -0.5 * magnitude * (1 + dev_uniform(st));
+        pert = 0.5 * magnitude * (1 + dev_uniform(st));
+        int cnt = (i < n) ? (cp[i + 1] - cp[i]) : 1;
+        int kk = cnt > 10 ? 9 : (cnt == 0 ? 0 : cnt - 1);
+        pert = wts[kk] * pert;
+        if (pert != 0.0)
+          for (int guard = 0; guard < 4000 && (fabs(pert) > fabs(max_p) || fabs(pert) < fabs(min_p)); ++guard) {
+            if (fabs(pert) > fabs(max_p)) pert = 0.1 * pert;
+            else if (fabs(pert) < fabs(min_p)) pert = 10 * pert;
+          }
+        c[i] += pert;
+      }
+      sh->rng = st;
+      sh->perturbed = 1;
+    }
+    sync();
+  }
+
+  // One basis change shared by both phases (sequentialDualSimplex.cpp:174-227).
+  __device__ void change_basis(int phase, double thP) {
+    const int p_idx = sh->p_idx, q = sh->q, q_idx = sh->q_idx, p = sh->p;
+    const double ap = aq[p_idx];
+    const double bp = beta[p_idx];
+    rowdot(rho, tau);
+    for (int i = tid; i < m; i += BS) {
+      double ai = aq[i];
+      if (phase == 2) {
+        int j = basis[i];
+        x[j] = x[j] - thP * ai;
+      } else if (i != p_idx) {
+        f[i] = f[i] - ai * thP;
+      }
+      if (i != p_idx) {
+        double r = ai / ap;
+        beta[i] = beta[i] - 2 * ai / ap * tau[i] + (r * r) * bp;
+      }
+    }
+    for (int cc = tid; cc < m; cc += BS) prow[cc] = Binv[p_idx * m + cc] / ap;
+    sync();
+    for (int i = 0; i < m; ++i) {
+      const double ai = aq[i];
+      for (int cc = tid; cc < m; cc += BS)
+        Binv[i * m + cc] = (i == p_idx) ? prow[cc] : Binv[i * m + cc] - ai * prow[cc];
+    }
+    if (tid == 0) {
+      beta[p_idx] = bp / (ap * ap);
+      if (phase == 2) x[q] = x[q] + thP;
+      else f[p_idx] = f[p_idx] / ap + sh->corr;
+      basis[p_idx] = q;
+      nonbasis[q_idx] = p;
+      posN[q] = -1;
+      posN[p] = q_idx;
+    }
+    sync();
+  }
+
+  // BTRAN (row of B^{-1}) and the pivot row over the nonbasic columns.
+  __device__ void pivot_row() {
+    const int p_idx = sh->p_idx;
+    for (int r = tid; r < m; r += BS) rho[r] = Binv[p_idx * m + r];
+    sync();
+    for (int i = tid; i < n; i += BS) ar[i] = col_dot(nonbasis[i], rho, 1.0);
+    sync();
+  }
+
+  // Entering column by the order-free textbook rule: minimum ratio, then the
+  // largest |alpha| within EPS_Z of it, then the lowest position.
+  __device__ int chuzc(int phase) {
+    const int neg = sh->neg;
+    double mn = BIG;
+    int cnt = 0;
+    for (int i = tid; i < n; i += BS) {
+      double a = neg ? -ar[i] : ar[i];
+      int j = nonbasis[i];
+      if (!candidate(phase, j, a)) continue;
+      ++cnt;
+      mn = fmin(mn, d[j] / a);
+    }
+    mn = block_min<BS>(mn);
+    cnt = block_sumi<BS>(cnt);
+    if (cnt == 0) return -1;
+    double best = -1.0;
+    int bi = INT_MAX;
+    for (int i = tid; i < n; i += BS) {
+      double a = neg ? -ar[i] : ar[i];
+      int j = nonbasis[i];
+      if (!candidate(phase, j, a)) continue;
+      if (fabs(d[j] / a - mn) < EPS_Z) argmax_merge(best, bi, fabs(a), i);
+    }
+    block_argmax<BS>(best, bi);
+    return bi == INT_MAX ? -1 : bi;
+  }
+
+  __device__ bool candidate(int phase, int j, double a) const {
+    if (phase == 2) {
+      int8_t t = bt[j];
+      if (a > EPS_A) return fabs(x[j] - l[j]) < EPS_Z && (t == BT_LOWER || t == BT_BOXED);
+      if (a < -EPS_A) return fabs(x[j] - u[j]) < EPS_Z && (t == BT_UPPER || t == BT_BOXED);
+      return t == BT_FREE;
+    }
+    double dj = d[j];
+    return (dj >= 0 && a > EPS_A) || (dj <= 0 && a < -EPS_A);
+  }
+
+  // FTRAN of column q: alpha_q = B^{-1} a_q.
+  __device__ void ftran_q() {
+    const int q = sh->q;
+    for (int i = tid; i < m; i += BS) {
+      double s = 0.0;
+      if (q < n) {
+        for (int e = cp[q]; e < cp[q + 1]; ++e) s += Binv[i * m + ri[e]] * cv[e];
+      } else {
+        s = Binv[i * m + (q - n)];
+      }
+      aq[i] = s;
+    }
+    sync();
+  }
+
+  __device__ bool tick() {
+    if (tid == 0) sh->iteration += 1;
+    sync();
+    return !(max_iter > 0 && sh->iteration > max_iter);
+  }
+
+  __device__ void clear_blocked() {
+    if (tid == 0) sh->nblocked = 0;
+    sync();
+  }
+  __device__ bool is_blocked(int j) const {
+    for (int b = 0; b < sh->nblocked; ++b)
+      if (sh->blocked[b] == j) return true;
+    return false;
+  }
+
+  // minimizeDualInfeasibility (tpp:799-934).  Returns 0 Solved, 1 DualInfeas,
+  // 2 iteration limit, 3 singular basis.
+  __device__ int phase1() {
+    phase1_init();
+    if (tid == 0) sh->cycle = 0;
+    clear_blocked();
+    while (true) {
+      if (!tick()) return 2;
+      if (perturb_on && !sh->perturbed && sh->cycle > MAX_CYCLE) {
+        perturb_costs();
+        phase1_init();
+      }
+      if (sh->cycle > NEED_RESTART || sh->obj > BIG || isnan(sh->obj)) phase1_init();
+      if (sh->iteration % K == 0 && !reinvert()) return 3;
+      double bw = -1.0;
+      int bi = INT_MAX;
+      for (int i = tid; i < m; i += BS) {
+        int j = basis[i];
+        int8_t t = bt[j];
+        double fi = f[i];
+        if ((t == BT_LOWER && fi > EPS_F) || (t == BT_UPPER && fi < -EPS_F) || t == BT_BOXED || t == BT_FIXED) {
+          double wgt = (fi * fi) / beta[i];
+          if (wgt > 0.0 && !is_blocked(j)) argmax_merge(bw, bi, wgt, i);
+        }
+      }
+      block_argmax<BS>(bw, bi);
+      const bool found = bi != INT_MAX;
+      const int dinf = sh->dinf;
+      const double obj = sh->obj;
+      if (!found || dinf == 0) {
+        if (fabs(obj) < EPS_A || dinf == 0) return 0;
+        if (obj > 0) return 1;
+      }
+      if (!found || obj > EPS_BOUND) {
+        if (!reinvert()) return 3;
+        phase1_init();
+        init_beta(false);
+        clear_blocked();
+        continue;
+      }
+      if (tid == 0) {
+        sh->p_idx = bi;
+        sh->p = basis[bi];
+        sh->f_p = f[bi];
+        sh->neg = f[bi] > 0 ? 1 : 0;
+      }
+      sync();
+      pivot_row();
+      const int qi = chuzc(1);
+      if (qi < 0) {
+        if (tid == 0 && sh->nblocked < 64) sh->blocked[sh->nblocked++] = sh->p;
+        sync();
+        if (!reinvert()) return 3;
+        phase1_init();
+        continue;
+      }
+      clear_blocked();
+      if (tid == 0) {
+        int q = nonbasis[qi];
+        sh->q_idx = qi;
+        sh->q = q;
+        sh->theta = d[q] / ar[qi];
+        int8_t tq = bt[q];
+        double dq = d[q];
+        double corr = 0.0;
+        if ((tq == BT_UPPER || tq == BT_FREE) && dq > EPS_D) corr = 1.0;
+        else if ((tq == BT_LOWER || tq == BT_FREE) && dq < -EPS_D) corr = -1.0;
+        sh->corr = corr;
+      }
+      sync();
+      ftran_q();
+      const int p_idx = sh->p_idx;
+      if (fabs(aq[p_idx] - ar[sh->q_idx]) > EPS_R * (1 + fabs(aq[p_idx])) && !reinvert()) return 3;
+      update_duals(1);
+      change_basis(1, sh->f_p / aq[p_idx]);
+      if (tid == 0) sh->cycle = (fabs(sh->theta) < EPS_A) ? sh->cycle + 1 : 0;
+      sync();
+    }
+  }
+
+  // d_N -= theta alpha_r; d_p = -theta; d_q = 0; objective update; recount of
+  // dual infeasibilities over the next nonbasic set.
+  __device__ void update_duals(int phase) {
+    const double theta = sh->theta;
+    const int q_idx = sh->q_idx;
+    int cnt = 0;
+    for (int i = tid; i < n; i += BS) {
+      int j = nonbasis[i];
+      double dj = d[j] - theta * ar[i];
+      d[j] = dj;
+      if (i != q_idx) cnt += dual_infeas(bt[j], dj);
+    }
+    cnt = block_sumi<BS>(cnt);
+    if (tid == 0) {
+      d[sh->p] = -theta;
+      d[sh->q] = 0.0;
+      sh->dinf = cnt + dual_infeas(bt[sh->p], -theta);
+      if (phase == 2) sh->obj = sh->obj + theta * sh->delta;
+      else sh->obj = sh->obj - theta * sh->f_p;
+    }
+    sync();
+  }
+
+  __device__ bool dual_feasible() {
+    int cnt = 0;
+    for (int i = tid; i < n; i += BS) {
+      int j = nonbasis[i];
+      cnt += dual_infeas(bt[j], d[j]);
+    }
+    cnt = block_sumi<BS>(cnt);
+    return cnt == 0;
+  }
+
+  // callDualSolver (sequentialDualSimplex.cpp:253-515).  Returns the status.
+  __device__ int phase2() {
+    dual_init();
+    if (tid == 0) sh->cycle = 0;
+    sync();
+    int unbound_checked = 0;
+    while (true) {
+      if (!tick()) return DS_ITER;
+      if (sh->iteration % K == 0 && !reinvert()) return DS_SING;
+      if (sh->iteration % NEED_RESTART == 0 && !dual_feasible()) {
+        if (!reinvert()) return DS_SING;
+        int cyc = sh->cycle;
+        int st = phase1();
+        if (st >= 2) return st == 2 ? DS_ITER : DS_SING;
+        dual_init();
+        if (tid == 0) sh->cycle = cyc;
+        sync();
+      }
+      if (perturb_on && !sh->perturbed && sh->cycle > MAX_CYCLE) {
+        perturb_costs();
+        if (!reinvert()) return DS_SING;
+        dual_init();
+      }
+      double bw = -1.0;
+      int bi = INT_MAX, pinf = 0;
+      for (int i = tid; i < m; i += BS) {
+        int j = basis[i];
+        int8_t t = bt[j];
+        double xv = x[j];
+        pinf |= primal_infeas(t, xv, l[j], u[j]) ? 1 : 0;
+        double dl;
+        if (var_infeas(t, xv, l[j], u[j], dl) && fabs(dl) > DSE_FLOOR) {
+          double wgt = (dl * dl) / beta[i];
+          if (wgt > 0.0) argmax_merge(bw, bi, wgt, i);
+        }
+      }
+      block_argmax<BS>(bw, bi);
+      pinf = block_sumi<BS>(pinf);
+      if (!pinf) return DS_OPT;
+      if (bi == INT_MAX) {
+        if (!reinvert()) return DS_SING;
+        init_beta(false);
+        dual_init();
+        continue;
+      }
+      if (tid == 0) {
+        int j = basis[bi];
+        double dl;
+        var_infeas(bt[j], x[j], l[j], u[j], dl);
+        sh->p_idx = bi;
+        sh->p = j;
+        sh->delta = dl;
+        sh->neg = dl < 0 ? 1 : 0;
+      }
+      sync();
+      pivot_row();
+      const int qi = chuzc(2);
+      if (qi < 0) {
+        if (unbound_checked) return DS_UNB;
+        unbound_checked = 1;
+        if (!reinvert()) return DS_SING;
+        init_beta(false);
+        int st = phase1();
+        if (st >= 2) return st == 2 ? DS_ITER : DS_SING;
+        dual_init();
+        continue;
+      }
+      unbound_checked = 0;
+      if (tid == 0) {
+        int q = nonbasis[qi];
+        sh->q_idx = qi;
+        sh->q = q;
+        sh->theta = d[q] / ar[qi];
+        sh->tmp = (bt[q] == BT_BOXED && fabs(sh->delta) - (u[q] - l[q]) * fabs(ar[qi]) > 0) ? 1.0 : 0.0;
+      }
+      sync();
+      if (sh->tmp != 0.0) return DS_BFRT;
+      ftran_q();
+      const int p_idx = sh->p_idx;
+      if (fabs(aq[p_idx] - ar[sh->q_idx]) > EPS_R * (1 + fabs(aq[p_idx])) && !reinvert()) return DS_SING;
+      update_duals(2);
+      change_basis(2, sh->delta / aq[p_idx]);
+      if (tid == 0 && fabs(sh->theta) < EPS_A) sh->cycle = sh->cycle + 1;
+      sync();
+    }
+  }
+
+  static constexpr int DS_OPT = 0, DS_UNB = 1, DS_DINF = 2, DS_ITER = 3, DS_SING = 5, DS_BFRT = 6;
+};
+
+}  // namespace
+
+size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds) {
+  const int N = mmax + nmax;
+  int P2 = 1;
+  while (P2 < N) P2 <<= 1;
+  size_t dbl = (size_t)4 * N + (size_t)7 * mmax + nmax + P2;
+  size_t ints = (size_t)mmax + nmax + N + mmax;
+  size_t bytes = dbl * 8 + ints * 4 + sizeof(BShared) + 64;
+  if (inv_in_lds) bytes += (size_t)mmax * mmax * 8;
+  return bytes;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
+  extern __shared__ double lds_d[];
+  const int mmax = D.mmax, nmax = D.nmax, Nmax = mmax + nmax;
+  int P2 = 1;
+  while (P2 < Nmax) P2 <<= 1;
+  double* base = lds_d;
+  LpSolve<BS> S;
+  S.tid = threadIdx.x;
+  S.c = base;
+  S.x = S.c + Nmax;
+  S.d = S.x + Nmax;
+  S.wN = S.d + Nmax;
+  S.beta = S.wN + Nmax;
+  S.f = S.beta + mmax;
+  S.rho = S.f + mmax;
+  S.aq = S.rho + mmax;
+  S.tau = S.aq + mmax;
+  S.w = S.tau + mmax;
+  S.prow = S.w + mmax;
+  S.ar = S.prow + mmax;
+  S.sortbuf = S.ar + nmax;
+  double* after = S.sortbuf + P2;
+  if (D.inv_in_lds) {
+    S.Binv = after;
+    after += (size_t)mmax * mmax;
+  } else {
+    S.Binv = D.gscratch + (size_t)blockIdx.x * mmax * mmax;
+  }
+  int* ip = (int*)after;
+  S.basis = ip;
+  S.nonbasis = S.basis + mmax;
+  S.posN = S.nonbasis + nmax;
+  S.piv = S.posN + Nmax;
+  S.sh = (BShared*)(((uintptr_t)(S.piv + mmax) + 15) & ~(uintptr_t)15);
+  S.K = D.refactor_freq;
+  S.max_iter = D.max_iter;
+  S.perturb_on = D.perturb;
+  S.seed = D.seed;
+  __shared__ int s_item;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(D.next, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= D.order_count) break;
+    const int lp = D.order[item];
+    const int m = D.m[lp], n = D.n[lp], N = m + n;
+    S.m = m;
+    S.n = n;
+    S.N = N;
+    S.cp = D.colptr + D.cp_off[lp];
+    S.ri = D.rowidx + D.nz_off[lp];
+    S.cv = D.cval + D.nz_off[lp];
+    S.rp = D.rowptr + D.rp_off[lp];
+    S.ci = D.colidx + D.nz_off[lp];
+    S.rv = D.rval + D.nz_off[lp];
+    S.corig = D.c + D.vo[lp];
+    S.l = D.l + D.vo[lp];
+    S.u = D.u + D.vo[lp];
+    S.bt = D.bt + D.vo[lp];
+    // Slack basis, or the given starting basis.
+    for (int j = threadIdx.x; j < N; j += BS) {
+      S.c[j] = S.corig[j];
+      S.x[j] = 0.0;
+      S.d[j] = 0.0;
+      S.posN[j] = -1;
+    }
+    if (threadIdx.x == 0) {
+      BShared* sh = S.sh;
+      sh->status = ST_OK;
+      sh->iteration = 0;
+      sh->perturbed = 0;
+      sh->cycle = 0;
+      sh->nblocked = 0;
+      sh->sing = 0;
+      sh->rng = D.seed;
+      if (D.warm) {
+        const int* wb = D.warm + D.mo[lp];
+        for (int i = 0; i < m; ++i) S.basis[i] = wb[i];
+      } else {
+        for (int i = 0; i < m; ++i) S.basis[i] = n + i;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += BS) S.posN[S.basis[i]] = -2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int k = 0;
+      for (int j = 0; j < N; ++j)
+        if (S.posN[j] == -1) {
+          S.nonbasis[k] = j;
+          S.posN[j] = k++;
+        }
+      for (int j = 0; j < N; ++j)
+        if (S.posN[j] == -2) S.posN[j] = -1;
+    }
+    __syncthreads();
+    int status;
+    bool ok = S.reinvert();
+    if (!ok) {
+      status = 5;
+    } else {
+      if (D.perturb && S.perturb_need()) S.perturb_costs();
+      S.init_beta(true);
+      int st = S.phase1();
+      if (st == 1) status = 2;
+      else if (st == 2) status = 3;
+      else if (st == 3) status = 5;
+      else {
+        S.set_primal();
+        status = S.phase2();
+        if (S.sh->perturbed && (status == 0 || status == 1)) {
+          for (int j = threadIdx.x; j < N; j += BS) S.c[j] = S.corig[j];
+          __syncthreads();
+          S.dual_init();
+        }
+      }
+    }
+    // Results.
+    if (threadIdx.x == 0) {
+      D.status[lp] = status;
+      D.obj[lp] = S.sh->obj;
+      D.iters[lp] = S.sh->iteration;
+    }
+    for (int i = threadIdx.x; i < m; i += BS) D.basis_out[D.mo[lp] + i] = S.basis[i];
+    for (int j = threadIdx.x; j < N; j += BS) D.x_out[D.vo[lp] + j] = S.x[j];
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_reset(int* next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *next = 0;
+}
+
+void launch_batch(hipStream_t s, const BatchDev& D, int grid, int block) {
+  size_t lds = batch_lds_bytes(D.mmax, D.nmax, D.inv_in_lds);
+  switch (block) {
+    case 128: k_batch<128><<<grid, 128, lds, s>>>(D); break;
+    case 512: k_batch<512><<<grid, 512, lds, s>>>(D); break;
+    default: k_batch<256><<<grid, 256, lds, s>>>(D); break;
+  }
+}
+
+void launch_batch_reset(hipStream_t s, int* next) { k_batch_reset<<<1, 64, 0, s>>>(next); }
+
+}  // namespace pds

@@ -1,0 +1,51 @@
+// Batched small-LP solver: device view of a batch and host launchers.
+#pragma once
+#include "ds_common.hpp"
+
+namespace pds {
+
+// A batch of LPs  min c^T x, A x <= b, x >= 0  (packing form), each held in
+// CSC and CSR with per-LP offsets into concatenated arrays.  Variables of LP k
+// are its n_k structurals then its m_k slacks (computational form).
+struct BatchDev {
+  int count;
+  const int* m;
+  const int* n;
+  const long long* cp_off;   // start of LP k's CSC col_ptr (n+1 entries)
+  const long long* rp_off;   // start of LP k's CSR row_ptr (m+1 entries)
+  const long long* nz_off;   // start of LP k's nonzeros
+  const long long* mo;       // start of LP k's m-sized slices
+  const long long* vo;       // start of LP k's (n+m)-sized slices
+  const int* colptr;
+  const int* rowidx;
+  const double* cval;
+  const int* rowptr;
+  const int* colidx;
+  const double* rval;
+  const double* c;           // n+m per LP (slack costs 0)
+  const double* l;
+  const double* u;
+  const int8_t* bt;
+  const int* order;          // LP indices of this launch's class, in processing order
+  int order_count;
+  const int* warm;           // optional starting basis (m per LP), nullptr = slack basis
+  int* status;
+  double* obj;
+  int* iters;
+  int* basis_out;            // m per LP
+  double* x_out;             // n+m per LP
+  int* next;                 // work-queue counter of this launch
+  double* gscratch;          // per-workgroup global inverse (mmax^2) when not in LDS
+  int mmax, nmax;
+  int inv_in_lds;
+  int refactor_freq;
+  int max_iter;
+  int perturb;
+  unsigned long long seed;
+};
+
+size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds);
+void launch_batch(hipStream_t s, const BatchDev& D, int grid, int block);
+void launch_batch_reset(hipStream_t s, int* next);
+
+}  // namespace pds
