@@ -58,6 +58,61 @@ def cpu_baseline(lp, refactor_freq, warmup, steps):
                       % (warmup + 1, warmup + done, warmup + steps, warmup, cpu_model())}
 
 
+def _oracle_solve_one(args):
+    import time as _t
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as ora
+    from parallel_ds_amd import lp as lpmod
+    k = args
+    lp = lpmod.config3_instance(k)
+    t0 = _t.perf_counter()
+    r, _, _ = ora.solve(lp)
+    return _t.perf_counter() - t0, r.status, r.Z
+
+
+def batched_family(count, tune_sample, reps, cpu_sample):
+    """4,096 packing LPs, m ~ U{32..256}, n = 2m, density 10%, one persistent launch."""
+    from parallel_ds_amd import lp as lpmod
+    from parallel_ds_amd.batch import BatchSolver, default_batch_params
+    lps = [lpmod.config3_instance(k) for k in range(count)]
+    with BatchSolver(lps) as B:
+        best, tune_ms, tried = B.tune(sample=tune_sample, base=default_batch_params(max_iterations=100000))
+        best.use_graph = 1
+        first_ms = B.solve(best)
+        times = [B.replay() for _ in range(reps)]
+        st, obj, it, bases, _ = B.results()
+    import numpy as np
+    ms = float(np.median(times))
+    out = {"workload": "%d packing LPs, m~U{32..256}, n=2m, density 10%%, seeds 3000+k" % count,
+           "solves_per_s": round(count / (ms * 1e-3), 2), "ms_per_batch": round(ms, 3),
+           "first_launch_ms": round(first_ms, 3), "replays": reps, "optimal": int((st == 0).sum()),
+           "mean_iterations": round(float(it.mean()), 1),
+           "ms_per_iter_per_lp": round(ms / max(1, float(it.sum())) * count, 5),
+           "tuned": {"block": best.block, "lds_inv_max_m": best.lds_inv_max_m, "wgs_per_cu": best.wgs_per_cu,
+                     "candidates": tried, "sample": tune_sample, "sample_ms": round(tune_ms, 3)}}
+    try:
+        import json as _j
+        g = _j.load(open(os.path.join(ROOT, "tests", "golden", "highs_objectives.json")))["config3"]
+        rel = [abs(obj[k] - r["objective"]) / abs(r["objective"]) for k, r in enumerate(g) if k < count]
+        out["max_rel_err_vs_highs_first_%d" % len(rel)] = float(max(rel))
+    except Exception:
+        pass
+    if cpu_sample > 0:
+        import multiprocessing as mp
+        cores = min(16, os.cpu_count() or 1)
+        ks = list(range(cpu_sample))
+        t0 = time.perf_counter()
+        with mp.Pool(cores) as pool:
+            res = pool.map(_oracle_solve_one, ks)
+        wall = time.perf_counter() - t0
+        single = sum(r[0] for r in res)
+        out["cpu_baseline"] = {"value": round(cpu_sample / wall, 2), "unit": "solves/s", "cores": cores,
+                               "kind": "port", "single_thread_solves_per_s": round(cpu_sample / single, 2),
+                               "sample": "oracle on instances 0..%d of the same family, one process per core"
+                                         % (cpu_sample - 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +125,10 @@ def main():
     ap.add_argument("--refactor", type=int, default=64)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--batch-cpu-sample", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -129,6 +188,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(lp, args.refactor, args.warmup, timed)
+    if not args.no_batch:
+        out["batched"] = batched_family(args.batch_count, 256, args.batch_reps,
+                                        0 if args.no_cpu_baseline else args.batch_cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -28,6 +28,204 @@ __device__ inline double dev_uniform(unsigned long long& s) {
   return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
+
+// Dense kernels on the explicit inverse (row-major, m x m).  They take
+// __restrict__ pointers and keep several loads in flight per thread: with
+// plain pointers the compiler must assume a store to B may change the LDS
+// vectors and serialises every element behind the previous store.
+
+// Row of flat index e < m*m (exact for m <= 4096: e/m is at least 0.5/m away
+// from the next integer, far above the float rounding error).
+__device__ __forceinline__ int flat_row(int e, float rinv, float half) {
+  return (int)((float)e * rinv + half);
+}
+
+// One Gauss-Jordan / product-form step on B:
+//   row p    <- r
+//   row i!=p <- B[i] - a[i] r, with column zc read as zero (zc < 0: none).
+template <int BS>
+__device__ __forceinline__ void inv_eliminate(double* __restrict__ B, const double* __restrict__ a,
+                                              const double* __restrict__ r, int m, int p, int zc, int tid) {
+  const int mm = m * m;
+  const float rinv = 1.0f / (float)m, half = 0.5f * rinv;
+  for (int e0 = tid; e0 < mm; e0 += 4 * BS) {
+    double v[4], ai[4], rc[4];
+    bool piv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k * BS;
+      const bool in = e < mm;
+      const int i = flat_row(e, rinv, half), cc = e - i * m;
+      piv[k] = i == p;
+      v[k] = (in && cc != zc) ? B[e] : 0.0;
+      ai[k] = in ? a[i] : 0.0;
+      rc[k] = in ? r[cc] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + k * BS;
+      if (e < mm) B[e] = piv[k] ? rc[k] : v[k] - ai[k] * rc[k];
+    }
+  }
+}
+
+// out[i] = sum_r B[i][r] v[r]: one wavefront per row, two rows at a time.
+template <int BS>
+__device__ __forceinline__ void inv_matvec(const double* __restrict__ B, const double* __restrict__ v,
+                                           double* __restrict__ out, int m, int tid) {
+  constexpr int NW = BS / WAVE;
+  const int lane = tid % WAVE, wv = tid / WAVE;
+  for (int i = wv; i < m; i += 2 * NW) {
+    const int i2 = i + NW;
+    const bool two = i2 < m;
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
+    for (int r = lane; r < m; r += WAVE) {
+      const double vr = v[r];
+      s0 += B[i * m + r] * vr;
+      if (two) s1 += B[i2 * m + r] * vr;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane == 0) {
+      out[i] = s0;
+      if (two) out[i2] = s1;
+    }
+  }
+}
+
+// out[i] = sum_k B[i][cmap[idx[k]]] val[k] over the nz entries of one
+// column: one wavefront per row, four rows in flight, the column's (mapped)
+// entries held in registers across rows.
+template <int BS>
+__device__ __forceinline__ void inv_spcol(const double* __restrict__ B, const int* __restrict__ cmap,
+                                          const int* __restrict__ idx, const double* __restrict__ val, int nz,
+                                          double* __restrict__ out, int m, int tid) {
+  constexpr int NW = BS / WAVE;
+  const int lane = tid % WAVE, wv = tid / WAVE;
+  if (nz <= 2 * WAVE) {
+    int ix[2];
+    double vx[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = lane + k * WAVE;
+      ix[k] = e < nz ? cmap[idx[e]] : 0;
+      vx[k] = e < nz ? val[e] : 0.0;
+    }
+    for (int i0 = wv; i0 < m; i0 += 4 * NW) {
+      double s[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r * NW;
+        s[r] = 0.0;
+        if (i < m) {
+          const double* Bi = B + (size_t)i * m;
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            if (lane + k * WAVE < nz) s[r] += Bi[ix[k]] * vx[k];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r * NW;
+        const double t = wave_sum(s[r]);
+        if (lane == 0 && i < m) out[i] = t;
+      }
+    }
+  } else {
+    for (int i = wv; i < m; i += NW) {
+      const double* Bi = B + (size_t)i * m;
+      double s = 0.0;
+      for (int e = lane; e < nz; e += WAVE) s += Bi[cmap[idx[e]]] * val[e];
+      s = wave_sum(s);
+      if (lane == 0) out[i] = s;
+    }
+  }
+}
+
+// The basis change on B in one pass: for every row i, tau[i] = B[i] . rs
+// from the old row, then row p <- pr and row i != p <- B[i] - a[i] pr.
+// One wavefront per row, four rows in flight (m <= 256 held in registers).
+template <int BS>
+__device__ __forceinline__ void inv_update_fused(double* __restrict__ B, const double* __restrict__ rs,
+                                                 const double* __restrict__ pr, const double* __restrict__ a,
+                                                 double* __restrict__ tau, int m, int p, int tid) {
+  constexpr int NW = BS / WAVE;
+  const int lane = tid % WAVE, wv = tid / WAVE;
+  if (m <= 4 * WAVE) {
+    double rr[4], pp[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = lane + k * WAVE;
+      rr[k] = c < m ? rs[c] : 0.0;
+      pp[k] = c < m ? pr[c] : 0.0;
+    }
+    for (int i0 = wv; i0 < m; i0 += 4 * NW) {
+      double v[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r * NW;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int c = lane + k * WAVE;
+          v[r][k] = (i < m && c < m) ? B[(size_t)i * m + c] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r * NW;
+        if (i >= m) continue;
+        double s = 0.0;
+        const double ai = a[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int c = lane + k * WAVE;
+          s += v[r][k] * rr[k];
+          if (c < m) B[(size_t)i * m + c] = (i == p) ? pp[k] : v[r][k] - ai * pp[k];
+        }
+        s = wave_sum(s);
+        if (lane == 0) tau[i] = s;
+      }
+    }
+  } else {
+    for (int i = wv; i < m; i += NW) {
+      double s = 0.0;
+      const double ai = a[i];
+      for (int c = lane; c < m; c += WAVE) {
+        const double v = B[(size_t)i * m + c];
+        s += v * rs[c];
+        B[(size_t)i * m + c] = (i == p) ? pr[c] : v - ai * pr[c];
+      }
+      s = wave_sum(s);
+      if (lane == 0) tau[i] = s;
+    }
+  }
+}
+
+// out[r] = sum_i v[i] B[i][r]: one column per thread, coalesced across threads.
+template <int BS>
+__device__ __forceinline__ void inv_tmatvec(const double* __restrict__ B, const double* __restrict__ v,
+                                            double* __restrict__ out, int m, int tid) {
+  for (int r = tid; r < m; r += BS) {
+    double y = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < m; ++i) y += v[i] * B[i * m + r];
+    out[r] = y;
+  }
+}
+
+// acc_j = alpha * sum_e val[e] v[idx[e]] over column j of a CSC matrix, in
+// CSC order, one column per thread.
+__device__ __forceinline__ double csc_dot(const int* __restrict__ cp, const int* __restrict__ ri,
+                                          const double* __restrict__ cv, const double* __restrict__ v, int j,
+                                          double alpha) {
+  double acc = 0.0;
+  const int e1 = cp[j + 1];
+#pragma unroll 4
+  for (int e = cp[j]; e < e1; ++e) acc = acc + cv[e] * v[ri[e]] * alpha;
+  return acc;
+}
+
 struct BShared {
   int status;
   int iteration;
@@ -46,6 +244,9 @@ struct BShared {
 
 constexpr int ST_OK = -1;  // internal: keep iterating
 
+// Every member is force-inlined into k_batch: as an out-of-line call the
+// object would live in scratch memory and each field would be reloaded after
+// every store through a generic pointer.
 template <int BS>
 struct LpSolve {
   int m, n, N;
@@ -76,102 +277,129 @@ struct LpSolve {
   int* nonbasis;
   int* posN;
   int* piv;
+  int* cmap;   // B^{-1}[i][c] = Binv[i * m + cmap[c]]
+  int* rpos;
+  int* rflag;
   double* Binv;
   BShared* sh;
   int K, max_iter, perturb_on;
   unsigned long long seed;
   int tid;
 
-  __device__ void sync() { __syncthreads(); }
+  __device__ __forceinline__ void sync() { __syncthreads(); }
 
-  // out[i] = sum_r Binv[i][r] * v[r], one wavefront per row (coalesced).
-  __device__ void rowdot(const double* v, double* out) {
-    const int lane = tid % WAVE, wv = tid / WAVE;
-    for (int i = wv; i < m; i += BS / WAVE) {
-      double s = 0.0;
-      for (int r = lane; r < m; r += WAVE) s += Binv[i * m + r] * v[r];
-      s = wave_sum(s);
-      if (lane == 0) out[i] = s;
-    }
+  // out = B^{-1} v for v indexed by constraint row (prow is the staging
+  // buffer in storage column order).
+  __device__ __forceinline__ void rowdot(const double* v, double* out) {
+    for (int c = tid; c < m; c += BS) prow[cmap[c]] = v[c];
+    sync();
+    inv_matvec<BS>(Binv, prow, out, m, tid);
     sync();
   }
 
   // Column j of [A | I] dotted with v, in CSC order (matrix.cpp:273-287 with alpha).
-  __device__ double col_dot(int j, const double* v, double alpha) const {
+  __device__ __forceinline__ double col_dot(int j, const double* v, double alpha) const {
     double acc = 0.0;
     if (j < n) {
-      for (int e = cp[j]; e < cp[j + 1]; ++e) acc = acc + cv[e] * v[ri[e]] * alpha;
+      acc = csc_dot(cp, ri, cv, v, j, alpha);
     } else {
       acc = acc + 1.0 * v[j - n] * alpha;
     }
     return acc;
   }
 
-  // B^{-1} from the current basis: Gauss-Jordan with partial pivoting.
-  __device__ bool reinvert() {
-    for (int i = 0; i < m; ++i)
-      for (int cc = tid; cc < m; cc += BS) Binv[i * m + cc] = 0.0;
+  // B^{-1} of the current basis by in-place Gauss-Jordan with partial
+  // pivoting, columns taken slacks first.  A slack column is a unit vector,
+  // so pivoting on it only exchanges two rows: thread 0 composes those
+  // exchanges into the row placement rpos[] used to build B, and only the
+  // structural columns cost an elimination pass.  The column exchanges an
+  // in-place Gauss-Jordan inverse ends with are not carried out; they are
+  // folded into cmap[] (B^{-1}[i][c] = Binv[i][cmap[c]]).
+  __device__ __forceinline__ bool reinvert() {
+    const int mm = m * m;
+    if (tid == 0) {
+      int* at = rflag;  // original row held at each position, slack pass only
+      for (int r = 0; r < m; ++r) {
+        rpos[r] = r;
+        at[r] = r;
+      }
+      for (int s = 0; s < m; ++s) {
+        const int j = basis[s];
+        if (j < n) continue;
+        const int r = j - n, cur = rpos[r];
+        piv[s] = cur;
+        if (cur != s) {
+          const int r2 = at[s];
+          at[s] = r;
+          at[cur] = r2;
+          rpos[r] = s;
+          rpos[r2] = cur;
+        }
+      }
+      for (int s = 0; s < m; ++s) rflag[s] = basis[s] >= n ? 1 : 0;
+    }
+    for (int e = tid; e < mm; e += BS) Binv[e] = 0.0;
     sync();
     for (int i = tid; i < m; i += BS) {
-      int j = basis[i];
+      const int j = basis[i];
       if (j < n) {
-        for (int e = cp[j]; e < cp[j + 1]; ++e) Binv[ri[e] * m + i] = cv[e];
+        for (int e = cp[j]; e < cp[j + 1]; ++e) Binv[rpos[ri[e]] * m + i] = cv[e];
       } else {
-        Binv[(j - n) * m + i] = 1.0;
+        Binv[rpos[j - n] * m + i] = 1.0;
       }
     }
     sync();
     for (int s = 0; s < m; ++s) {
+      if (basis[s] >= n) continue;
       double bv = -1.0;
       int bi = INT_MAX;
-      for (int r = s + tid; r < m; r += BS) {
-        double a = fabs(Binv[r * m + s]);
-        if (a >= 0.0) argmax_merge(bv, bi, a, r);
-      }
+      for (int r = tid; r < m; r += BS)
+        if (!rflag[r]) {
+          const double a = fabs(Binv[r * m + s]);
+          if (a >= 0.0) argmax_merge(bv, bi, a, r);
+        }
       block_argmax<BS>(bv, bi);
       if (bi == INT_MAX || !(bv > 0.0)) {
         if (tid == 0) sh->sing = 1;
         sync();
         return false;
       }
-      if (tid == 0) piv[s] = bi;
-      if (bi != s)
-        for (int cc = tid; cc < m; cc += BS) {
-          double a = Binv[s * m + cc];
-          Binv[s * m + cc] = Binv[bi * m + cc];
-          Binv[bi * m + cc] = a;
-        }
-      sync();
-      const double pv = Binv[s * m + s];
-      sync();
-      for (int cc = tid; cc < m; cc += BS) Binv[s * m + cc] = ((cc == s) ? 1.0 : Binv[s * m + cc]) / pv;
-      for (int i = tid; i < m; i += BS) w[i] = (i == s) ? 0.0 : Binv[i * m + s];
-      sync();
-      for (int i = 0; i < m; ++i) {
-        if (i == s) continue;
-        const double wi = w[i];
-        for (int cc = tid; cc < m; cc += BS) {
-          double a = (cc == s) ? 0.0 : Binv[i * m + cc];
-          Binv[i * m + cc] = a - wi * Binv[s * m + cc];
-        }
+      if (tid == 0) {
+        piv[s] = bi;
+        rflag[s] = 1;
       }
+      // Swap rows s and bi while reading the pivot row into prow.  Column s
+      // of row bi is left alone here (the pivot value is read from it by every
+      // thread); the elimination below overwrites it.
+      const double pv = Binv[bi * m + s];
+      for (int cc = tid; cc < m; cc += BS) {
+        double a = Binv[s * m + cc];
+        double b = Binv[bi * m + cc];
+        if (bi != s && cc != s) Binv[bi * m + cc] = a;
+        prow[cc] = ((cc == s) ? 1.0 : b) / pv;
+      }
+      for (int i = tid; i < m; i += BS) w[i] = (i == s) ? 0.0 : ((i == bi) ? Binv[s * m + s] : Binv[i * m + s]);
+      sync();
+      inv_eliminate<BS>(Binv, w, prow, m, s, s, tid);
       sync();
     }
-    for (int s = m - 1; s >= 0; --s) {
-      int r = piv[s];
-      if (r != s)
-        for (int i = tid; i < m; i += BS) {
-          double a = Binv[i * m + s];
-          Binv[i * m + s] = Binv[i * m + r];
-          Binv[i * m + r] = a;
+    if (tid == 0) {
+      for (int c = 0; c < m; ++c) cmap[c] = c;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int s = m - 1; s >= 0; --s) {
+          const bool slack = basis[s] >= n;
+          if (slack != (pass == 1) || piv[s] == s) continue;
+          const int t = cmap[piv[s]];
+          cmap[piv[s]] = cmap[s];
+          cmap[s] = t;
         }
-      sync();
     }
+    sync();
     return true;
   }
 
   // initBetaWeights (tpp:34-70): identity or exact row norms of B^{-1}.
-  __device__ void init_beta(bool identity) {
+  __device__ __forceinline__ void init_beta(bool identity) {
     if (identity) {
       for (int i = tid; i < m; i += BS) beta[i] = 1.0;
     } else {
@@ -187,12 +415,12 @@ struct LpSolve {
   }
 
   // y = B^{-T} c_B ; d_N = c_N - A_N^T y  (sequentialDualSimplex.cpp:119-132)
-  __device__ void init_reduced_costs() {
-    for (int r = tid; r < m; r += BS) {
-      double y = 0.0;
-      for (int i = 0; i < m; ++i) y += c[basis[i]] * Binv[i * m + r];
-      w[r] = y;
-    }
+  __device__ __forceinline__ void init_reduced_costs() {
+    for (int i = tid; i < m; i += BS) tau[i] = c[basis[i]];
+    sync();
+    inv_tmatvec<BS>(Binv, tau, prow, m, tid);
+    sync();
+    for (int r = tid; r < m; r += BS) w[r] = prow[cmap[r]];
     sync();
     for (int i = tid; i < n; i += BS) {
       int j = nonbasis[i];
@@ -202,7 +430,7 @@ struct LpSolve {
   }
 
   // setPrimalVars (tpp:74-101)
-  __device__ void set_primal() {
+  __device__ __forceinline__ void set_primal() {
     for (int i = tid; i < n; i += BS) {
       int j = nonbasis[i];
       double v = 0.0;
@@ -219,7 +447,7 @@ struct LpSolve {
   }
 
   // dualSimplexInit (sequentialDualSimplex.cpp:230-250)
-  __device__ void dual_init() {
+  __device__ __forceinline__ void dual_init() {
     set_primal();
     for (int r = tid; r < m; r += BS) {
       double s = 0.0 * 1.0;
@@ -245,7 +473,7 @@ struct LpSolve {
   }
 
   // minimizeDualInfeasibilityInit + initPhase1PricingVector
-  __device__ void phase1_init() {
+  __device__ __forceinline__ void phase1_init() {
     init_reduced_costs();
     double z = 0.0;
     int cnt = 0;
@@ -280,7 +508,7 @@ struct LpSolve {
 
   // checkPerturbNeed (tpp:223-234): bitonic sort of the costs, then the
   // tolerance-unique count of the sorted sequence.
-  __device__ bool perturb_need() {
+  __device__ __forceinline__ bool perturb_need() {
     int P2 = 1;
     while (P2 < N) P2 <<= 1;
     for (int j = tid; j < P2; j += BS) sortbuf[j] = (j < N) ? c[j] : 1e300;
@@ -315,7 +543,7 @@ struct LpSolve {
   }
 
   // perturbCosts (tpp:238-275) with the seeded stream.
-  __device__ void perturb_costs() {
+  __device__ __forceinline__ void perturb_costs() {
     double s = 0.0;
     for (int j = tid; j < N; j += BS) s += c[j];
     s = block_sum<BS>(s);
@@ -347,11 +575,20 @@ struct LpSolve {
   }
 
   // One basis change shared by both phases (sequentialDualSimplex.cpp:174-227).
-  __device__ void change_basis(int phase, double thP) {
+  __device__ __forceinline__ void change_basis(int phase, double thP) {
     const int p_idx = sh->p_idx, q = sh->q, q_idx = sh->q_idx, p = sh->p;
     const double ap = aq[p_idx];
     const double bp = beta[p_idx];
-    rowdot(rho, tau);
+    // Row p of B^{-1} in storage order (w) and divided by alpha_p (prow);
+    // tau = B^{-1} rho is formed from the old rows inside the update pass.
+    for (int cc = tid; cc < m; cc += BS) {
+      const double v = Binv[p_idx * m + cc];
+      w[cc] = v;
+      prow[cc] = v / ap;
+    }
+    sync();
+    inv_update_fused<BS>(Binv, w, prow, aq, tau, m, p_idx, tid);
+    sync();
     for (int i = tid; i < m; i += BS) {
       double ai = aq[i];
       if (phase == 2) {
@@ -365,13 +602,7 @@ struct LpSolve {
         beta[i] = beta[i] - 2 * ai / ap * tau[i] + (r * r) * bp;
       }
     }
-    for (int cc = tid; cc < m; cc += BS) prow[cc] = Binv[p_idx * m + cc] / ap;
     sync();
-    for (int i = 0; i < m; ++i) {
-      const double ai = aq[i];
-      for (int cc = tid; cc < m; cc += BS)
-        Binv[i * m + cc] = (i == p_idx) ? prow[cc] : Binv[i * m + cc] - ai * prow[cc];
-    }
     if (tid == 0) {
       beta[p_idx] = bp / (ap * ap);
       if (phase == 2) x[q] = x[q] + thP;
@@ -385,9 +616,9 @@ struct LpSolve {
   }
 
   // BTRAN (row of B^{-1}) and the pivot row over the nonbasic columns.
-  __device__ void pivot_row() {
+  __device__ __forceinline__ void pivot_row() {
     const int p_idx = sh->p_idx;
-    for (int r = tid; r < m; r += BS) rho[r] = Binv[p_idx * m + r];
+    for (int r = tid; r < m; r += BS) rho[r] = Binv[p_idx * m + cmap[r]];
     sync();
     for (int i = tid; i < n; i += BS) ar[i] = col_dot(nonbasis[i], rho, 1.0);
     sync();
@@ -395,7 +626,7 @@ struct LpSolve {
 
   // Entering column by the order-free textbook rule: minimum ratio, then the
   // largest |alpha| within EPS_Z of it, then the lowest position.
-  __device__ int chuzc(int phase) {
+  __device__ __forceinline__ int chuzc(int phase) {
     const int neg = sh->neg;
     double mn = BIG;
     int cnt = 0;
@@ -421,7 +652,7 @@ struct LpSolve {
     return bi == INT_MAX ? -1 : bi;
   }
 
-  __device__ bool candidate(int phase, int j, double a) const {
+  __device__ __forceinline__ bool candidate(int phase, int j, double a) const {
     if (phase == 2) {
       int8_t t = bt[j];
       if (a > EPS_A) return fabs(x[j] - l[j]) < EPS_Z && (t == BT_LOWER || t == BT_BOXED);
@@ -433,31 +664,28 @@ struct LpSolve {
   }
 
   // FTRAN of column q: alpha_q = B^{-1} a_q.
-  __device__ void ftran_q() {
+  __device__ __forceinline__ void ftran_q() {
     const int q = sh->q;
-    for (int i = tid; i < m; i += BS) {
-      double s = 0.0;
-      if (q < n) {
-        for (int e = cp[q]; e < cp[q + 1]; ++e) s += Binv[i * m + ri[e]] * cv[e];
-      } else {
-        s = Binv[i * m + (q - n)];
-      }
-      aq[i] = s;
+    if (q < n) {
+      inv_spcol<BS>(Binv, cmap, ri + cp[q], cv + cp[q], cp[q + 1] - cp[q], aq, m, tid);
+    } else {
+      const int cq = cmap[q - n];
+      for (int i = tid; i < m; i += BS) aq[i] = Binv[i * m + cq];
     }
     sync();
   }
 
-  __device__ bool tick() {
+  __device__ __forceinline__ bool tick() {
     if (tid == 0) sh->iteration += 1;
     sync();
     return !(max_iter > 0 && sh->iteration > max_iter);
   }
 
-  __device__ void clear_blocked() {
+  __device__ __forceinline__ void clear_blocked() {
     if (tid == 0) sh->nblocked = 0;
     sync();
   }
-  __device__ bool is_blocked(int j) const {
+  __device__ __forceinline__ bool is_blocked(int j) const {
     for (int b = 0; b < sh->nblocked; ++b)
       if (sh->blocked[b] == j) return true;
     return false;
@@ -465,7 +693,7 @@ struct LpSolve {
 
   // minimizeDualInfeasibility (tpp:799-934).  Returns 0 Solved, 1 DualInfeas,
   // 2 iteration limit, 3 singular basis.
-  __device__ int phase1() {
+  __device__ __forceinline__ int phase1() {
     phase1_init();
     if (tid == 0) sh->cycle = 0;
     clear_blocked();
@@ -545,7 +773,7 @@ struct LpSolve {
 
   // d_N -= theta alpha_r; d_p = -theta; d_q = 0; objective update; recount of
   // dual infeasibilities over the next nonbasic set.
-  __device__ void update_duals(int phase) {
+  __device__ __forceinline__ void update_duals(int phase) {
     const double theta = sh->theta;
     const int q_idx = sh->q_idx;
     int cnt = 0;
@@ -566,7 +794,7 @@ struct LpSolve {
     sync();
   }
 
-  __device__ bool dual_feasible() {
+  __device__ __forceinline__ bool dual_feasible() {
     int cnt = 0;
     for (int i = tid; i < n; i += BS) {
       int j = nonbasis[i];
@@ -577,7 +805,7 @@ struct LpSolve {
   }
 
   // callDualSolver (sequentialDualSimplex.cpp:253-515).  Returns the status.
-  __device__ int phase2() {
+  __device__ __forceinline__ int phase2() {
     dual_init();
     if (tid == 0) sh->cycle = 0;
     sync();
@@ -670,10 +898,11 @@ struct LpSolve {
 
 size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds) {
   const int N = mmax + nmax;
-  int P2 = 1;
-  while (P2 < N) P2 <<= 1;
-  size_t dbl = (size_t)4 * N + (size_t)7 * mmax + nmax + P2;
-  size_t ints = (size_t)mmax + nmax + N + mmax;
+  // c, x, d, wN (N each); beta, f, rho, aq, tau, w, prow (m each); ar (n).
+  // The sort buffer of checkPerturbNeed aliases wN..ar.
+  size_t dbl = (size_t)4 * N + (size_t)7 * mmax + nmax;
+  // basis (m), nonbasis (n), posN (N), piv, cmap, rpos, rflag (m each).
+  size_t ints = (size_t)mmax + nmax + N + 4 * (size_t)mmax;
   size_t bytes = dbl * 8 + ints * 4 + sizeof(BShared) + 64;
   if (inv_in_lds) bytes += (size_t)mmax * mmax * 8;
   return bytes;
@@ -683,8 +912,6 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   extern __shared__ double lds_d[];
   const int mmax = D.mmax, nmax = D.nmax, Nmax = mmax + nmax;
-  int P2 = 1;
-  while (P2 < Nmax) P2 <<= 1;
   double* base = lds_d;
   LpSolve<BS> S;
   S.tid = threadIdx.x;
@@ -700,8 +927,11 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   S.w = S.tau + mmax;
   S.prow = S.w + mmax;
   S.ar = S.prow + mmax;
-  S.sortbuf = S.ar + nmax;
-  double* after = S.sortbuf + P2;
+  // checkPerturbNeed runs once per LP before wN, beta, f, rho, aq, tau, w,
+  // prow and ar hold anything; its sort buffer (a power of two below
+  // 2(m+n) <= m+n + 7m + n doubles) reuses that region.
+  S.sortbuf = S.wN;
+  double* after = S.ar + nmax;
   if (D.inv_in_lds) {
     S.Binv = after;
     after += (size_t)mmax * mmax;
@@ -713,7 +943,10 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   S.nonbasis = S.basis + mmax;
   S.posN = S.nonbasis + nmax;
   S.piv = S.posN + Nmax;
-  S.sh = (BShared*)(((uintptr_t)(S.piv + mmax) + 15) & ~(uintptr_t)15);
+  S.cmap = S.piv + mmax;
+  S.rpos = S.cmap + mmax;
+  S.rflag = S.rpos + mmax;
+  S.sh = (BShared*)(((uintptr_t)(S.rflag + mmax) + 15) & ~(uintptr_t)15);
   S.K = D.refactor_freq;
   S.max_iter = D.max_iter;
   S.perturb_on = D.perturb;

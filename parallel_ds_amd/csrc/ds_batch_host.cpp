@@ -321,7 +321,7 @@ void BatchSolver::tune(int sample, ds_batch_params& best, double& best_ms, int& 
   const int step = std::max(1, count_ / std::max(1, sample));
   for (int k = 0; k < count_ && (int)items.size() < sample; k += step) items.push_back(k);
   const int blocks[3] = {128, 256, 512};
-  const int ldsm[3] = {0, 64, 96};
+  const int ldsm[3] = {0, 96, 128};
   const int occ[3] = {1, 2, 4};
   std::vector<int> ref_st(count_);
   std::vector<double> ref_obj(count_);
