@@ -160,6 +160,58 @@ int ds_batch_results(ds_ctx* ctx, int* status, double* obj, int* iters, int* bas
  * `sample` LPs; writes the fastest into *best (its other fields are kept). */
 int ds_batch_tune(ds_ctx* ctx, int sample, ds_batch_params* best, double* best_ms, int* tried);
 
+/* ---- Learned warm start ----
+ * No counterpart in the reference (its solver always starts from the slack
+ * basis, sequentialDualSimplex.cpp:230-250).  A per-variable MLP (16 features
+ * -> 3 hidden layers of width 256 -> logit of "basic at the optimum"; BF16 on
+ * the matrix cores, FP32 accumulation and master weights) scores every
+ * variable of every LP of a batch; each LP starts from its m best-scored
+ * variables, repaired by slack substitution in its first factorization;
+ * the dual simplex then certifies optimality as from any basis. */
+typedef struct ds_mlp ds_mlp;
+
+int ds_mlp_create(int device, int n_features /* 1..32 */, uint64_t seed, ds_mlp** out);
+int ds_mlp_destroy(ds_mlp* net);
+const char* ds_mlp_last_error(ds_mlp* net);
+long long ds_mlp_param_count(void);
+/* FP32 parameter vector: W1[256x32] b1[256] W2[256x256] b2 W3[256x256] b3 w4[256] b4. */
+int ds_mlp_get_params(ds_mlp* net, float* params);
+int ds_mlp_set_params(ds_mlp* net, const float* params);
+/* Feature standardisation (x - mu) / sd applied before the first layer. */
+int ds_mlp_set_norm(ds_mlp* net, const float* mu, const float* sd);
+int ds_mlp_get_norm(ds_mlp* net, float* mu, float* sd);
+/* Logits of S feature rows (S x n_features, host buffers). */
+int ds_mlp_infer(ds_mlp* net, int S, const float* features, float* logits);
+/* Mean weighted BCE-with-logits loss of S rows and its gradient w.r.t. the
+ * parameter vector (host buffers; weights may be NULL = all 1). */
+int ds_mlp_grad(ds_mlp* net, int S, const float* features, const float* labels, const float* weights, double* loss,
+                float* grad);
+int ds_mlp_adam_step(ds_mlp* net, float lr);  /* one Adam step on the last gradient */
+/* Trains on the batch's last solved bases: features and labels of every
+ * variable of every LP, standardisation fitted to them, `epochs` shuffled
+ * passes of Adam with minibatches of `minibatch` rows.  epoch_loss (may be
+ * NULL) receives the mean loss of each epoch, *ms the device time. */
+int ds_mlp_fit(ds_mlp* net, ds_ctx* batch, int epochs, int minibatch, float lr, uint64_t seed, double* epoch_loss,
+               double* ms);
+/* Device time (ms) and MFMA flops of the training GEMMs so far. */
+int ds_mlp_train_stats(ds_mlp* net, double* ms, long long* flops);
+
+/* Per-variable features (sum(n_k+m_k) x 16, FP32) and 0/1 labels of the last
+ * solved bases (sum(n_k+m_k)), host copies. */
+int ds_batch_features(ds_ctx* ctx, float* features);
+int ds_batch_labels(ds_ctx* ctx, float* labels);
+/* Starting bases from per-variable scores (sum(n_k+m_k), host): the m_k best
+ * of each LP.  ds_batch_warm_start does the same on the device from the
+ * network's scores; *ms = features + inference + selection wall time,
+ * *infer_ms = the inference kernel alone. */
+int ds_batch_warm_from_scores(ds_ctx* ctx, const float* scores);
+int ds_batch_warm_start(ds_ctx* ctx, ds_mlp* net, double* ms, double* infer_ms);
+int ds_batch_get_warm(ds_ctx* ctx, int* basis /* sum m */);
+/* Pivot magnitude below which the first factorization of a starting basis
+ * substitutes a slack (default 1e-7; 0 disables: singular bases fail). */
+int ds_batch_set_repair_tol(ds_ctx* ctx, double tol);
+int ds_batch_repairs(ds_ctx* ctx, int* count /* per LP */);
+
 /* Trailing-update tile check: C(64x64, row-major) -= P(:, :nb) * B(nb x 64), where
  * P is 64x64 row-major (columns >= nb ignored) and B is nb x 64 row-major; the
  * FP64 MFMA kernel when use_mfma, else the plain FMA kernel. */

@@ -285,6 +285,7 @@ struct Solver {
   Factor F;
   std::vector<Eta> etas;
   bool perturbed = false;
+  bool warm = false;
   int iteration = 0;
   double obj = 0.0;
   int status = ORA_OPTIMAL;
@@ -304,6 +305,22 @@ struct Solver {
     for (int i = 0; i < N - m; ++i) nonbasis[i] = i;
     for (int i = 0; i < m; ++i) basis[i] = i + (N - m);
     if (P.refactor_freq <= 0) P.refactor_freq = 150;
+  }
+
+  /* Starting basis other than the slack basis: the m given variables, the
+   * nonbasic set in ascending order (no counterpart in the reference, whose
+   * solver always starts from the slack basis). */
+  void set_start(const int* b) {
+    std::vector<char> in(N, 0);
+    for (int i = 0; i < m; ++i) {
+      if (b[i] < 0 || b[i] >= N || in[b[i]]) throw std::runtime_error("invalid starting basis");
+      in[b[i]] = 1;
+      basis[i] = b[i];
+    }
+    int k = 0;
+    for (int j = 0; j < N; ++j)
+      if (!in[j]) nonbasis[k++] = j;
+    warm = true;
   }
 
   void tick() {
@@ -877,7 +894,7 @@ struct Solver {
   void run(double& Z) {
     reFactorize();
     if (P.perturb && checkPerturbNeed()) perturbCosts();
-    initBetaWeights(true);
+    initBetaWeights(!warm);  /* exact weights for a given starting basis */
     int st = phase1();
     if (st == 1) {
       status = ORA_DUAL_INFEASIBLE;
@@ -903,12 +920,22 @@ extern "C" int ora_solve(int m, int n, const int* col_ptr, const int* row_idx, c
                          const int* bound_type, const double* lower_range, const double* upper_range,
                          const int* range_type, const ora_params* params, ora_result* result,
                          double* x_out, int* basis_out) {
+  return ora_solve_from(m, n, col_ptr, row_idx, val, costs, lower_bound, upper_bound, bound_type, lower_range,
+                        upper_range, range_type, nullptr, params, result, x_out, basis_out);
+}
+
+extern "C" int ora_solve_from(int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                              const double* costs, const double* lower_bound, const double* upper_bound,
+                              const int* bound_type, const double* lower_range, const double* upper_range,
+                              const int* range_type, const int* basis_in, const ora_params* params,
+                              ora_result* result, double* x_out, int* basis_out) {
   try {
     Model M;
     build_model(M, m, n, col_ptr, row_idx, val, costs, lower_bound, upper_bound, bound_type,
                 lower_range, upper_range, range_type);
     ora_params P = *params;
     Solver S(M, P);
+    if (basis_in) S.set_start(basis_in);
     double Z = 0.0;
     try {
       S.run(Z);

@@ -59,6 +59,15 @@ int ora_solve(int m, int n, const int* col_ptr, const int* row_idx, const double
               const int* bound_type, const double* lower_range, const double* upper_range,
               const int* range_type, const ora_params* params,
               ora_result* result, double* x_out, int* basis_out);
+/* ora_solve from the starting basis basis_in (m variable ids, nonbasic set in
+ * ascending order, exact DSE weights); NULL = the slack basis.  The reference
+ * has no such entry: it replays the same algorithm from another basis, as the
+ * batched solver's warm starts do. */
+int ora_solve_from(int m, int n, const int* col_ptr, const int* row_idx, const double* val,
+                   const double* costs, const double* lower_bound, const double* upper_bound,
+                   const int* bound_type, const double* lower_range, const double* upper_range,
+                   const int* range_type, const int* basis_in, const ora_params* params, ora_result* result,
+                   double* x_out, int* basis_out);
 
 /* Matrix::dotUpdate (matrix.cpp:261-346).  A is m x ncols in CSC+CSR form built
  * from the given CSR.  method 0 = UPDATE (row-wise), 1 = UPDATE_T (column-wise). */

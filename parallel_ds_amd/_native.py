@@ -79,6 +79,26 @@ _SIGS = {
     "ds_batch_replay": (I32, [VP, C.POINTER(F64)]),
     "ds_batch_results": (I32, [VP, VP, VP, VP, VP, VP]),
     "ds_batch_tune": (I32, [VP, I32, C.POINTER(BatchParams), C.POINTER(F64), C.POINTER(I32)]),
+    "ds_mlp_create": (I32, [I32, I32, U64, C.POINTER(VP)]),
+    "ds_mlp_destroy": (I32, [VP]),
+    "ds_mlp_last_error": (C.c_char_p, [VP]),
+    "ds_mlp_param_count": (C.c_longlong, []),
+    "ds_mlp_get_params": (I32, [VP, VP]),
+    "ds_mlp_set_params": (I32, [VP, VP]),
+    "ds_mlp_set_norm": (I32, [VP, VP, VP]),
+    "ds_mlp_get_norm": (I32, [VP, VP, VP]),
+    "ds_mlp_infer": (I32, [VP, I32, VP, VP]),
+    "ds_mlp_grad": (I32, [VP, I32, VP, VP, VP, C.POINTER(F64), VP]),
+    "ds_mlp_adam_step": (I32, [VP, C.c_float]),
+    "ds_mlp_fit": (I32, [VP, VP, I32, I32, C.c_float, U64, VP, C.POINTER(F64)]),
+    "ds_mlp_train_stats": (I32, [VP, C.POINTER(F64), C.POINTER(C.c_longlong)]),
+    "ds_batch_features": (I32, [VP, VP]),
+    "ds_batch_labels": (I32, [VP, VP]),
+    "ds_batch_warm_from_scores": (I32, [VP, VP]),
+    "ds_batch_warm_start": (I32, [VP, VP, C.POINTER(F64), C.POINTER(F64)]),
+    "ds_batch_get_warm": (I32, [VP, VP]),
+    "ds_batch_set_repair_tol": (I32, [VP, F64]),
+    "ds_batch_repairs": (I32, [VP, VP]),
     "ds_gen_bernoulli_lp": (I32, [I32, I32, F64, U64, VP, VP, VP, I32, VP, VP]),
     "ds_gen_kcol_lp": (I32, [I32, I32, I32, U64, VP, VP, VP, VP, VP]),
     "ds_gen_transport": (I32, [I32, I32, U64, VP, VP, VP, VP, VP]),

@@ -101,3 +101,47 @@ class BatchSolver:
         self._check(lib().ds_batch_tune(self._ctx, sample, C.byref(best), C.byref(ms), C.byref(tried)),
                     "ds_batch_tune")
         return best, ms.value, tried.value
+
+    # ---- learned warm start -------------------------------------------------
+    @property
+    def nvars(self):
+        return int(self.voff[-1])
+
+    def features(self):
+        """Per-variable features, (sum(n_k+m_k), 16) FP32, computed on the device."""
+        F = np.zeros((self.nvars, 16), np.float32)
+        self._check(lib().ds_batch_features(self._ctx, ptr(F)), "ds_batch_features")
+        return F
+
+    def labels(self):
+        """1.0 for the variables of each LP's last solved basis, else 0.0."""
+        y = np.zeros(self.nvars, np.float32)
+        self._check(lib().ds_batch_labels(self._ctx, ptr(y)), "ds_batch_labels")
+        return y
+
+    def warm_from_scores(self, scores):
+        s = np.ascontiguousarray(scores, np.float32)
+        if s.shape != (self.nvars,):
+            raise ValueError("one score per variable expected")
+        self._check(lib().ds_batch_warm_from_scores(self._ctx, ptr(s)), "ds_batch_warm_from_scores")
+
+    def warm_start(self, net):
+        """Starting bases from the network (features, inference, top-m on the device).
+        Returns (wall ms of the three steps, inference kernel ms)."""
+        ms, ims = C.c_double(), C.c_double()
+        self._check(lib().ds_batch_warm_start(self._ctx, net._h, C.byref(ms), C.byref(ims)), "ds_batch_warm_start")
+        return ms.value, ims.value
+
+    def get_warm(self):
+        b = np.zeros(int(self.moff[-1]), np.int32)
+        self._check(lib().ds_batch_get_warm(self._ctx, ptr(b)), "ds_batch_get_warm")
+        return [b[self.moff[k]:self.moff[k + 1]] for k in range(self.count)]
+
+    def set_repair_tol(self, tol):
+        self._check(lib().ds_batch_set_repair_tol(self._ctx, float(tol)), "ds_batch_set_repair_tol")
+
+    def repairs(self):
+        r = np.zeros(self.count, np.int32)
+        self._check(lib().ds_batch_repairs(self._ctx, ptr(r)), "ds_batch_repairs")
+        return r
+

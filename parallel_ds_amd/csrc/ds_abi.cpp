@@ -5,12 +5,20 @@
 
 #include "ds_abi.h"
 #include "ds_batch_host.hpp"
+#include "ds_learn.hpp"
 #include "ds_solver.hpp"
+#include "ds_warm.hpp"
 
 struct ds_ctx {
   pds::Solver* solver = nullptr;
   pds::BatchSolver* batch = nullptr;
+  pds::WarmBuffers* wb = nullptr;
   int device = 0;
+  std::string err;
+};
+
+struct ds_mlp {
+  pds::Mlp* net = nullptr;
   std::string err;
 };
 
@@ -43,6 +51,27 @@ int guard(ds_ctx* ctx, F&& f) {
     return DS_ESTATE;
   } catch (const std::exception& e) {
     ctx->err = e.what();
+    return DS_EINTERNAL;
+  }
+}
+
+template <typename F>
+int mguard(ds_mlp* m, F&& f) {
+  if (!m || !m->net) return DS_EINVAL;
+  try {
+    f(*m->net);
+    return DS_OK;
+  } catch (const pds::HipError& e) {
+    m->err = e.what();
+    return DS_EHIP;
+  } catch (const std::invalid_argument& e) {
+    m->err = e.what();
+    return DS_EINVAL;
+  } catch (const std::logic_error& e) {
+    m->err = e.what();
+    return DS_ESTATE;
+  } catch (const std::exception& e) {
+    m->err = e.what();
     return DS_EINTERNAL;
   }
 }
@@ -92,6 +121,7 @@ int ds_create(int device, ds_ctx** out) {
 
 int ds_destroy(ds_ctx* ctx) {
   if (!ctx) return DS_EINVAL;
+  delete ctx->wb;
   delete ctx->batch;
   delete ctx->solver;
   delete ctx;
@@ -285,6 +315,175 @@ int ds_batch_tune(ds_ctx* ctx, int sample, ds_batch_params* best, double* best_m
     if (best_ms) *best_ms = ms;
     if (tried) *tried = t;
   });
+}
+
+int ds_mlp_create(int device, int n_features, uint64_t seed, ds_mlp** out) {
+  if (!out) return DS_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return DS_EHIP;
+  if (device < 0 || device >= ndev || n_features <= 0 || n_features > pds::MLP_KPAD) return DS_EINVAL;
+  ds_mlp* m = new ds_mlp;
+  try {
+    m->net = new pds::Mlp(device, n_features, seed);
+  } catch (const std::exception&) {
+    delete m;
+    return DS_EHIP;
+  }
+  *out = m;
+  return DS_OK;
+}
+
+int ds_mlp_destroy(ds_mlp* net) {
+  if (!net) return DS_EINVAL;
+  delete net->net;
+  delete net;
+  return DS_OK;
+}
+
+const char* ds_mlp_last_error(ds_mlp* net) { return net ? net->err.c_str() : "null network"; }
+
+long long ds_mlp_param_count(void) { return (long long)pds::MlpLayout::total; }
+
+int ds_mlp_get_params(ds_mlp* net, float* params) {
+  if (!params) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) { M.get_params(params); });
+}
+
+int ds_mlp_set_params(ds_mlp* net, const float* params) {
+  if (!params) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) { M.set_params(params); });
+}
+
+int ds_mlp_set_norm(ds_mlp* net, const float* mu, const float* sd) {
+  if (!mu || !sd) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) { M.set_norm(mu, sd); });
+}
+
+int ds_mlp_get_norm(ds_mlp* net, float* mu, float* sd) {
+  if (!mu || !sd) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) { M.get_norm(mu, sd); });
+}
+
+int ds_mlp_infer(ds_mlp* net, int S, const float* features, float* logits) {
+  if (S <= 0 || !features || !logits) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) {
+    pds::DBuf<float> F, z;
+    F.alloc((size_t)S * M.n_features());
+    z.alloc(S);
+    PDS_HIP(hipMemcpyAsync(F.p, features, sizeof(float) * S * M.n_features(), hipMemcpyHostToDevice, M.stream()));
+    M.infer(F.p, nullptr, S, z.p);
+    PDS_HIP(hipMemcpyAsync(logits, z.p, sizeof(float) * S, hipMemcpyDeviceToHost, M.stream()));
+    PDS_HIP(hipStreamSynchronize(M.stream()));
+  });
+}
+
+int ds_mlp_grad(ds_mlp* net, int S, const float* features, const float* labels, const float* weights, double* loss,
+                float* grad) {
+  if (S <= 0 || !features || !labels || !grad) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) {
+    pds::DBuf<float> F, y, w;
+    F.alloc((size_t)S * M.n_features());
+    y.alloc(S);
+    PDS_HIP(hipMemcpyAsync(F.p, features, sizeof(float) * S * M.n_features(), hipMemcpyHostToDevice, M.stream()));
+    PDS_HIP(hipMemcpyAsync(y.p, labels, sizeof(float) * S, hipMemcpyHostToDevice, M.stream()));
+    if (weights) {
+      w.alloc(S);
+      PDS_HIP(hipMemcpyAsync(w.p, weights, sizeof(float) * S, hipMemcpyHostToDevice, M.stream()));
+    }
+    const double l = M.grad(F.p, nullptr, S, y.p, weights ? w.p : nullptr);
+    if (loss) *loss = l;
+    PDS_HIP(hipMemcpyAsync(grad, M.grad_dev(), sizeof(float) * pds::MlpLayout::total, hipMemcpyDeviceToHost,
+                           M.stream()));
+    PDS_HIP(hipStreamSynchronize(M.stream()));
+  });
+}
+
+int ds_mlp_adam_step(ds_mlp* net, float lr) {
+  return mguard(net, [&](pds::Mlp& M) {
+    M.adam_step(lr);
+    PDS_HIP(hipStreamSynchronize(M.stream()));
+  });
+}
+
+int ds_mlp_fit(ds_mlp* net, ds_ctx* batch, int epochs, int minibatch, float lr, uint64_t seed, double* epoch_loss,
+               double* ms) {
+  if (!net || !net->net || !batch || !batch->batch) return DS_EINVAL;
+  if (epochs <= 0 || minibatch <= 0 || !(lr > 0.f)) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) {
+    if (!batch->wb) batch->wb = new pds::WarmBuffers;
+    const double t = pds::fit_from_batch(M, *batch->batch, *batch->wb, epochs, minibatch, lr, seed, epoch_loss);
+    if (ms) *ms = t;
+  });
+}
+
+int ds_mlp_train_stats(ds_mlp* net, double* ms, long long* flops) {
+  return mguard(net, [&](pds::Mlp& M) {
+    if (ms) *ms = M.gemm_ms();
+    if (flops) *flops = M.gemm_flops();
+  });
+}
+
+int ds_batch_features(ds_ctx* ctx, float* features) {
+  if (!features) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    if (B.count() == 0) throw std::logic_error("no batch loaded");
+    pds::DBuf<float> F;
+    F.alloc((size_t)B.vars() * pds::WS_NF);
+    B.features(F.p);
+    PDS_HIP(hipMemcpyAsync(features, F.p, sizeof(float) * B.vars() * pds::WS_NF, hipMemcpyDeviceToHost,
+                           B.stream()));
+    PDS_HIP(hipStreamSynchronize(B.stream()));
+  });
+}
+
+int ds_batch_labels(ds_ctx* ctx, float* labels) {
+  if (!labels) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    if (B.count() == 0) throw std::logic_error("no batch loaded");
+    pds::DBuf<float> y;
+    y.alloc(B.vars());
+    B.labels(y.p);
+    PDS_HIP(hipMemcpyAsync(labels, y.p, sizeof(float) * B.vars(), hipMemcpyDeviceToHost, B.stream()));
+    PDS_HIP(hipStreamSynchronize(B.stream()));
+  });
+}
+
+int ds_batch_warm_from_scores(ds_ctx* ctx, const float* scores) {
+  if (!scores) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    if (B.count() == 0) throw std::logic_error("no batch loaded");
+    if (!ctx->wb) ctx->wb = new pds::WarmBuffers;
+    ctx->wb->z.alloc(B.vars());
+    PDS_HIP(hipMemcpyAsync(ctx->wb->z.p, scores, sizeof(float) * B.vars(), hipMemcpyHostToDevice, B.stream()));
+    B.select_warm(ctx->wb->z.p);
+    PDS_HIP(hipStreamSynchronize(B.stream()));
+  });
+}
+
+int ds_batch_warm_start(ds_ctx* ctx, ds_mlp* net, double* ms, double* infer_ms) {
+  if (!net || !net->net) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) {
+    if (B.count() == 0) throw std::logic_error("no batch loaded");
+    if (!ctx->wb) ctx->wb = new pds::WarmBuffers;
+    const double t = pds::warm_start_batch(*net->net, B, *ctx->wb, infer_ms);
+    if (ms) *ms = t;
+  });
+}
+
+int ds_batch_get_warm(ds_ctx* ctx, int* basis) {
+  if (!basis) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.get_warm(basis); });
+}
+
+int ds_batch_set_repair_tol(ds_ctx* ctx, double tol) {
+  if (!(tol >= 0.0)) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.set_repair_tol(tol); });
+}
+
+int ds_batch_repairs(ds_ctx* ctx, int* count) {
+  if (!count) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.repairs(count); });
 }
 
 int ds_debug_gemm_tile(double* C, const double* P, const double* B, int nb, int use_mfma) {

@@ -143,22 +143,27 @@ __device__ __forceinline__ void inv_spcol(const double* __restrict__ B, const in
   }
 }
 
-// The basis change on B in one pass: for every row i, tau[i] = B[i] . rs
-// from the old row, then row p <- pr and row i != p <- B[i] - a[i] pr.
-// One wavefront per row, four rows in flight (m <= 256 held in registers).
-template <int BS>
+// The basis change on B in one pass.  For every row i, from the old row:
+//   FT:  a[i] = B[i] . ad   (FTRAN of the entering column, ad dense in storage
+//        column order; a[p] is set to ap, computed the same way beforehand)
+//   tau[i] = B[i] . rs      (rs = old row p: the DSE vector B^{-1} rho)
+// then row p <- pr and row i != p <- B[i] - a[i] pr.  One wavefront per row,
+// four rows in flight; rows of m <= 256 are held in registers.
+template <int BS, bool FT>
 __device__ __forceinline__ void inv_update_fused(double* __restrict__ B, const double* __restrict__ rs,
-                                                 const double* __restrict__ pr, const double* __restrict__ a,
-                                                 double* __restrict__ tau, int m, int p, int tid) {
+                                                 const double* __restrict__ pr, const double* __restrict__ ad,
+                                                 double ap, double* __restrict__ a, double* __restrict__ tau,
+                                                 int m, int p, int tid) {
   constexpr int NW = BS / WAVE;
   const int lane = tid % WAVE, wv = tid / WAVE;
   if (m <= 4 * WAVE) {
-    double rr[4], pp[4];
+    double rr[4], pp[4], dd[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int c = lane + k * WAVE;
       rr[k] = c < m ? rs[c] : 0.0;
       pp[k] = c < m ? pr[c] : 0.0;
+      dd[k] = (FT && c < m) ? ad[c] : 0.0;
     }
     for (int i0 = wv; i0 < m; i0 += 4 * NW) {
       double v[4][4];
@@ -175,8 +180,17 @@ __device__ __forceinline__ void inv_update_fused(double* __restrict__ B, const d
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + r * NW;
         if (i >= m) continue;
+        double ai;
+        if (FT) {
+          double sa = 0.0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) sa += v[r][k] * dd[k];
+          sa = wave_sum(sa);
+          ai = (i == p) ? ap : sa;
+        } else {
+          ai = a[i];
+        }
         double s = 0.0;
-        const double ai = a[i];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const int c = lane + k * WAVE;
@@ -184,22 +198,44 @@ __device__ __forceinline__ void inv_update_fused(double* __restrict__ B, const d
           if (c < m) B[(size_t)i * m + c] = (i == p) ? pp[k] : v[r][k] - ai * pp[k];
         }
         s = wave_sum(s);
-        if (lane == 0) tau[i] = s;
+        if (lane == 0) {
+          tau[i] = s;
+          if (FT) a[i] = ai;
+        }
       }
     }
   } else {
     for (int i = wv; i < m; i += NW) {
+      double ai = 0.0;
+      if (FT) {
+        double sa = 0.0;
+        for (int c = lane; c < m; c += WAVE) sa += B[(size_t)i * m + c] * ad[c];
+        sa = wave_sum(sa);
+        ai = (i == p) ? ap : sa;
+      } else {
+        ai = a[i];
+      }
       double s = 0.0;
-      const double ai = a[i];
       for (int c = lane; c < m; c += WAVE) {
         const double v = B[(size_t)i * m + c];
         s += v * rs[c];
         B[(size_t)i * m + c] = (i == p) ? pr[c] : v - ai * pr[c];
       }
       s = wave_sum(s);
-      if (lane == 0) tau[i] = s;
+      if (lane == 0) {
+        tau[i] = s;
+        if (FT) a[i] = ai;
+      }
     }
   }
+}
+
+// v . d over m entries in the lane order the fused pass uses for one row.
+__device__ __forceinline__ double wave_rowdot(const double* __restrict__ v, const double* __restrict__ d, int m,
+                                              int lane) {
+  double s = 0.0;
+  for (int c = lane; c < m; c += WAVE) s += v[c] * d[c];
+  return wave_sum(s);
 }
 
 // out[r] = sum_i v[i] B[i][r]: one column per thread, coalesced across threads.
@@ -237,6 +273,7 @@ struct BShared {
   int dinf;
   int sing;
   int nblocked;
+  int nrep;
   int blocked[64];
   double delta, theta, obj, f_p, corr, tmp;
   unsigned long long rng;
@@ -280,6 +317,9 @@ struct LpSolve {
   int* cmap;   // B^{-1}[i][c] = Binv[i * m + cmap[c]]
   int* rpos;
   int* rflag;
+  int* rat;
+  int repair_on;       // slack substitution for (near-)singular columns
+  double repair_tol;
   double* Binv;
   BShared* sh;
   int K, max_iter, perturb_on;
@@ -318,7 +358,7 @@ struct LpSolve {
   __device__ __forceinline__ bool reinvert() {
     const int mm = m * m;
     if (tid == 0) {
-      int* at = rflag;  // original row held at each position, slack pass only
+      int* at = rat;  // original row held at each position
       for (int r = 0; r < m; ++r) {
         rpos[r] = r;
         at[r] = r;
@@ -359,6 +399,28 @@ struct LpSolve {
           if (a >= 0.0) argmax_merge(bv, bi, a, r);
         }
       block_argmax<BS>(bv, bi);
+      if (repair_on && !(bi != INT_MAX && bv > repair_tol)) {
+        // Slack substitution: column s depends (numerically) on the columns
+        // already pivoted.  Every position below s is used (slack positions by
+        // the slack pass, structural ones by their steps), so position s is the
+        // lowest free one; the slack of the original row held there enters.
+        // Transformed by the steps so far, that slack's column is the unit
+        // vector at s: row operations only add multiples of pivot rows, and
+        // the row at s has not been one.
+        if (tid == 0) {
+          const int jold = basis[s], jnew = n + rat[s];
+          const int k = posN[jnew];
+          nonbasis[k] = jold;
+          posN[jold] = k;
+          posN[jnew] = -1;
+          basis[s] = jnew;
+          sh->nrep += 1;
+        }
+        for (int i = tid; i < m; i += BS) Binv[i * m + s] = (i == s) ? 1.0 : 0.0;
+        bi = s;
+        bv = 1.0;
+        sync();
+      }
       if (bi == INT_MAX || !(bv > 0.0)) {
         if (tid == 0) sh->sing = 1;
         sync();
@@ -367,6 +429,9 @@ struct LpSolve {
       if (tid == 0) {
         piv[s] = bi;
         rflag[s] = 1;
+        const int t = rat[s];
+        rat[s] = rat[bi];
+        rat[bi] = t;
       }
       // Swap rows s and bi while reading the pivot row into prow.  Column s
       // of row bi is left alone here (the pivot value is read from it by every
@@ -575,20 +640,47 @@ struct LpSolve {
   }
 
   // One basis change shared by both phases (sequentialDualSimplex.cpp:174-227).
-  __device__ __forceinline__ void change_basis(int phase, double thP) {
-    const int p_idx = sh->p_idx, q = sh->q, q_idx = sh->q_idx, p = sh->p;
-    const double ap = aq[p_idx];
-    const double bp = beta[p_idx];
-    // Row p of B^{-1} in storage order (w) and divided by alpha_p (prow);
-    // tau = B^{-1} rho is formed from the old rows inside the update pass.
-    for (int cc = tid; cc < m; cc += BS) {
-      const double v = Binv[p_idx * m + cc];
-      w[cc] = v;
-      prow[cc] = v / ap;
+  // alpha_p = B^{-1}[p] . a_q with the entering column scattered densely in
+  // storage order into rho (free once the pivot row is formed) and row p of
+  // B^{-1} staged in w; the result goes to sh->tmp.
+  __device__ __forceinline__ double pivot_alpha() {
+    const int q = sh->q, p_idx = sh->p_idx;
+    for (int c = tid; c < m; c += BS) {
+      rho[c] = 0.0;
+      w[c] = Binv[p_idx * m + c];
     }
     sync();
-    inv_update_fused<BS>(Binv, w, prow, aq, tau, m, p_idx, tid);
+    if (q < n) {
+      for (int e = cp[q] + tid; e < cp[q + 1]; e += BS) rho[cmap[ri[e]]] = cv[e];
+    } else if (tid == 0) {
+      rho[cmap[q - n]] = 1.0;
+    }
     sync();
+    if (tid < WAVE) {
+      const double ap = wave_rowdot(w, rho, m, tid);
+      if (tid == 0) sh->tmp = ap;
+    }
+    sync();
+    return sh->tmp;
+  }
+
+  // Basis change: FTRAN (when FT; otherwise aq holds alpha_q), DSE vector,
+  // product-form update of B^{-1} in one pass, then the primal / phase-1 and
+  // weight updates (sequentialDualSimplex.cpp:174-227).
+  template <bool FT>
+  __device__ __forceinline__ void change_basis(int phase, double ap_in) {
+    const int p_idx = sh->p_idx, q = sh->q, q_idx = sh->q_idx, p = sh->p;
+    if (!FT) {
+      for (int cc = tid; cc < m; cc += BS) w[cc] = Binv[p_idx * m + cc];
+      sync();
+    }
+    const double ap = FT ? ap_in : aq[p_idx];
+    const double bp = beta[p_idx];
+    for (int cc = tid; cc < m; cc += BS) prow[cc] = w[cc] / ap;
+    sync();
+    inv_update_fused<BS, FT>(Binv, w, prow, rho, ap, aq, tau, m, p_idx, tid);
+    sync();
+    const double thP = (phase == 2) ? sh->delta / ap : sh->f_p / ap;
     for (int i = tid; i < m; i += BS) {
       double ai = aq[i];
       if (phase == 2) {
@@ -761,11 +853,17 @@ struct LpSolve {
         sh->corr = corr;
       }
       sync();
-      ftran_q();
-      const int p_idx = sh->p_idx;
-      if (fabs(aq[p_idx] - ar[sh->q_idx]) > EPS_R * (1 + fabs(aq[p_idx])) && !reinvert()) return 3;
-      update_duals(1);
-      change_basis(1, sh->f_p / aq[p_idx]);
+      const double ap = pivot_alpha();
+      if (fabs(ap - ar[sh->q_idx]) > EPS_R * (1 + fabs(ap))) {
+        // Unstable pivot: alpha_q from the old inverse is kept across the refactor.
+        ftran_q();
+        if (!reinvert()) return 3;
+        update_duals(1);
+        change_basis<false>(1, 0.0);
+      } else {
+        update_duals(1);
+        change_basis<true>(1, ap);
+      }
       if (tid == 0) sh->cycle = (fabs(sh->theta) < EPS_A) ? sh->cycle + 1 : 0;
       sync();
     }
@@ -881,11 +979,17 @@ struct LpSolve {
       }
       sync();
       if (sh->tmp != 0.0) return DS_BFRT;
-      ftran_q();
-      const int p_idx = sh->p_idx;
-      if (fabs(aq[p_idx] - ar[sh->q_idx]) > EPS_R * (1 + fabs(aq[p_idx])) && !reinvert()) return DS_SING;
-      update_duals(2);
-      change_basis(2, sh->delta / aq[p_idx]);
+      const double ap = pivot_alpha();
+      if (fabs(ap - ar[sh->q_idx]) > EPS_R * (1 + fabs(ap))) {
+        // Unstable pivot: alpha_q from the old inverse is kept across the refactor.
+        ftran_q();
+        if (!reinvert()) return DS_SING;
+        update_duals(2);
+        change_basis<false>(2, 0.0);
+      } else {
+        update_duals(2);
+        change_basis<true>(2, ap);
+      }
       if (tid == 0 && fabs(sh->theta) < EPS_A) sh->cycle = sh->cycle + 1;
       sync();
     }
@@ -901,8 +1005,8 @@ size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds) {
   // c, x, d, wN (N each); beta, f, rho, aq, tau, w, prow (m each); ar (n).
   // The sort buffer of checkPerturbNeed aliases wN..ar.
   size_t dbl = (size_t)4 * N + (size_t)7 * mmax + nmax;
-  // basis (m), nonbasis (n), posN (N), piv, cmap, rpos, rflag (m each).
-  size_t ints = (size_t)mmax + nmax + N + 4 * (size_t)mmax;
+  // basis (m), nonbasis (n), posN (N), piv, cmap, rpos, rflag, rat (m each).
+  size_t ints = (size_t)mmax + nmax + N + 5 * (size_t)mmax;
   size_t bytes = dbl * 8 + ints * 4 + sizeof(BShared) + 64;
   if (inv_in_lds) bytes += (size_t)mmax * mmax * 8;
   return bytes;
@@ -913,7 +1017,7 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   extern __shared__ double lds_d[];
   const int mmax = D.mmax, nmax = D.nmax, Nmax = mmax + nmax;
   double* base = lds_d;
-  LpSolve<BS> S;
+  LpSolve<BS> S = {};
   S.tid = threadIdx.x;
   S.c = base;
   S.x = S.c + Nmax;
@@ -946,8 +1050,11 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   S.cmap = S.piv + mmax;
   S.rpos = S.cmap + mmax;
   S.rflag = S.rpos + mmax;
-  S.sh = (BShared*)(((uintptr_t)(S.rflag + mmax) + 15) & ~(uintptr_t)15);
+  S.rat = S.rflag + mmax;
+  S.sh = (BShared*)(((uintptr_t)(S.rat + mmax) + 15) & ~(uintptr_t)15);
   S.K = D.refactor_freq;
+  S.repair_tol = D.repair_tol;
+  S.repair_on = 0;
   S.max_iter = D.max_iter;
   S.perturb_on = D.perturb;
   S.seed = D.seed;
@@ -987,7 +1094,9 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
       sh->perturbed = 0;
       sh->cycle = 0;
       sh->nblocked = 0;
+      sh->nrep = 0;
       sh->sing = 0;
+      sh->obj = 0.0;
       sh->rng = D.seed;
       if (D.warm) {
         const int* wb = D.warm + D.mo[lp];
@@ -1011,12 +1120,16 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
     }
     __syncthreads();
     int status;
+    // A given starting basis may be repaired by slack substitution in its
+    // first factorization; its DSE weights are the exact row norms.
+    S.repair_on = (D.warm && D.repair_tol > 0.0) ? 1 : 0;
     bool ok = S.reinvert();
+    S.repair_on = 0;
     if (!ok) {
       status = 5;
     } else {
       if (D.perturb && S.perturb_need()) S.perturb_costs();
-      S.init_beta(true);
+      S.init_beta(D.warm == nullptr);
       int st = S.phase1();
       if (st == 1) status = 2;
       else if (st == 2) status = 3;
@@ -1036,6 +1149,7 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
       D.status[lp] = status;
       D.obj[lp] = S.sh->obj;
       D.iters[lp] = S.sh->iteration;
+      if (D.repairs) D.repairs[lp] = S.sh->nrep;
     }
     for (int i = threadIdx.x; i < m; i += BS) D.basis_out[D.mo[lp] + i] = S.basis[i];
     for (int j = threadIdx.x; j < N; j += BS) D.x_out[D.vo[lp] + j] = S.x[j];

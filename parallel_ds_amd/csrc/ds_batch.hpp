@@ -29,6 +29,8 @@ struct BatchDev {
   const int* order;          // LP indices of this launch's class, in processing order
   int order_count;
   const int* warm;           // optional starting basis (m per LP), nullptr = slack basis
+  double repair_tol;         // > 0: slack substitution for pivots below it in the first factorization
+  int* repairs;              // per LP: number of substitutions (may be null)
   int* status;
   double* obj;
   int* iters;

@@ -4,6 +4,8 @@
 // a hipGraph), and tunes the launch shape.
 #include "ds_batch_host.hpp"
 
+#include "ds_warm.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -140,6 +142,9 @@ void BatchSolver::load(int count, const int* m, const int* n, const long long* n
   basis_.alloc(mso);
   x_.alloc(vso);
   next_.alloc(4);
+  repairs_.alloc(count);
+  solved_ = false;
+  PDS_HIP(hipMemsetAsync(repairs_.p, 0, sizeof(int) * count, st_));
   warm_on_ = false;
   PDS_HIP(hipStreamSynchronize(st_));
   drop_graph();
@@ -210,35 +215,53 @@ void BatchSolver::plan(const std::vector<int>& items, const ds_batch_params& p) 
   PDS_HIP(hipStreamSynchronize(st_));
 }
 
+BatchDev BatchSolver::make_dev() const {
+  BatchDev D;
+  D.count = count_;
+  D.m = m_.p;
+  D.n = n_.p;
+  D.cp_off = cp_off_.p;
+  D.rp_off = rp_off_.p;
+  D.nz_off = nz_off_.p;
+  D.mo = mo_.p;
+  D.vo = vo_.p;
+  D.colptr = colptr_.p;
+  D.rowidx = rowidx_.p;
+  D.cval = cval_.p;
+  D.rowptr = rowptr_.p;
+  D.colidx = colidx_.p;
+  D.rval = rval_.p;
+  D.c = c_.p;
+  D.l = l_.p;
+  D.u = u_.p;
+  D.bt = bt_.p;
+  D.order = nullptr;
+  D.order_count = 0;
+  D.warm = warm_on_ ? warm_.p : nullptr;
+  D.repair_tol = warm_on_ ? repair_tol_ : 0.0;
+  D.repairs = repairs_.p;
+  D.status = status_.p;
+  D.obj = obj_.p;
+  D.iters = iters_.p;
+  D.basis_out = basis_.p;
+  D.x_out = x_.p;
+  D.next = nullptr;
+  D.gscratch = gscratch_.p;
+  D.mmax = mmax_all_;
+  D.nmax = nmax_all_;
+  D.inv_in_lds = 0;
+  D.refactor_freq = 150;
+  D.max_iter = 0;
+  D.perturb = 0;
+  D.seed = 0;
+  return D;
+}
+
 void BatchSolver::enqueue(const ds_batch_params& p) {
   for (const Class& C : classes_) {
-    BatchDev D;
-    D.count = count_;
-    D.m = m_.p;
-    D.n = n_.p;
-    D.cp_off = cp_off_.p;
-    D.rp_off = rp_off_.p;
-    D.nz_off = nz_off_.p;
-    D.mo = mo_.p;
-    D.vo = vo_.p;
-    D.colptr = colptr_.p;
-    D.rowidx = rowidx_.p;
-    D.cval = cval_.p;
-    D.rowptr = rowptr_.p;
-    D.colidx = colidx_.p;
-    D.rval = rval_.p;
-    D.c = c_.p;
-    D.l = l_.p;
-    D.u = u_.p;
-    D.bt = bt_.p;
+    BatchDev D = make_dev();
     D.order = order_.p + C.first;
     D.order_count = C.count;
-    D.warm = warm_on_ ? warm_.p : nullptr;
-    D.status = status_.p;
-    D.obj = obj_.p;
-    D.iters = iters_.p;
-    D.basis_out = basis_.p;
-    D.x_out = x_.p;
     D.next = next_.p + C.counter;
     D.gscratch = gscratch_.p;
     D.mmax = C.mmax;
@@ -251,6 +274,28 @@ void BatchSolver::enqueue(const ds_batch_params& p) {
     launch_batch_reset(st_, D.next);
     launch_batch(st_, D, C.grid, p.block);
   }
+}
+
+void BatchSolver::features(float* F_dev) {
+  BatchDev D = make_dev();
+  launch_features(st_, D, count_, mmax_all_, nmax_all_, F_dev);
+  PDS_HIP(hipGetLastError());
+}
+
+void BatchSolver::labels(float* y_dev) {
+  if (!solved_) throw std::logic_error("labels need a solved batch");
+  BatchDev D = make_dev();
+  launch_labels(st_, D, count_, basis_.p, y_dev);
+  PDS_HIP(hipGetLastError());
+}
+
+void BatchSolver::select_warm(const float* z_dev) {
+  BatchDev D = make_dev();
+  warm_.alloc(std::max<long long>(sum_m_, 1));
+  launch_select(st_, D, count_, mmax_all_ + nmax_all_, z_dev, warm_.p);
+  PDS_HIP(hipGetLastError());
+  warm_on_ = true;
+  drop_graph();
 }
 
 double BatchSolver::run(const std::vector<int>& items, const ds_batch_params& p, bool use_graph) {
@@ -286,7 +331,9 @@ double BatchSolver::run(const std::vector<int>& items, const ds_batch_params& p,
 double BatchSolver::solve(const ds_batch_params& p) {
   std::vector<int> all(count_);
   for (int k = 0; k < count_; ++k) all[k] = k;
-  return run(all, p, p.use_graph != 0);
+  const double ms = run(all, p, p.use_graph != 0);
+  solved_ = true;
+  return ms;
 }
 
 double BatchSolver::replay() {
@@ -303,6 +350,17 @@ double BatchSolver::replay() {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ms;
+}
+
+void BatchSolver::get_warm(int* out) {
+  if (!warm_on_) throw std::logic_error("no starting bases set");
+  PDS_HIP(hipMemcpyAsync(out, warm_.p, sizeof(int) * sum_m_, hipMemcpyDeviceToHost, st_));
+  PDS_HIP(hipStreamSynchronize(st_));
+}
+
+void BatchSolver::repairs(int* out) {
+  PDS_HIP(hipMemcpyAsync(out, repairs_.p, sizeof(int) * count_, hipMemcpyDeviceToHost, st_));
+  PDS_HIP(hipStreamSynchronize(st_));
 }
 
 void BatchSolver::results(int* status, double* obj, int* iters, int* basis, double* x) {

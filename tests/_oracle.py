@@ -32,6 +32,9 @@ def lib():
         vp = C.c_void_p
         L.ora_solve.restype = C.c_int
         L.ora_solve.argtypes = [C.c_int, C.c_int] + [vp] * 10 + [C.POINTER(OraParams), C.POINTER(OraResult), vp, vp]
+        L.ora_solve_from.restype = C.c_int
+        L.ora_solve_from.argtypes = [C.c_int, C.c_int] + [vp] * 11 + [C.POINTER(OraParams), C.POINTER(OraResult),
+                                                                      vp, vp]
         L.ora_dot_update.restype = C.c_int
         L.ora_dot_update.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, vp, C.c_int,
                                      C.c_int, C.c_int]
@@ -62,6 +65,20 @@ def solve(lp, **kw):
     rc = lib().ora_solve(lp.m, lp.n, *[P(a) for a in lp.arrays()], C.byref(prm), C.byref(res), P(x), P(basis))
     if rc != 0:
         raise RuntimeError("ora_solve failed")
+    return res, x, basis
+
+
+def solve_from(lp, start, **kw):
+    """Oracle solve from the starting basis `start` (m variable ids)."""
+    prm = params(**kw)
+    res = OraResult()
+    x = np.zeros(lp.n + lp.m)
+    basis = np.zeros(lp.m, np.int32)
+    st = np.ascontiguousarray(start, np.int32)
+    rc = lib().ora_solve_from(lp.m, lp.n, *[P(a) for a in lp.arrays()], P(st), C.byref(prm), C.byref(res), P(x),
+                              P(basis))
+    if rc != 0:
+        raise RuntimeError("ora_solve_from failed")
     return res, x, basis
 
 
