@@ -29,6 +29,14 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel err≤1e-9"
 
 
+def log(msg):
+    """Progress on stderr (the JSON result is the only stdout line)."""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -75,8 +83,10 @@ def batched_family(count, tune_sample, reps, cpu_sample):
     from parallel_ds_amd import lp as lpmod
     from parallel_ds_amd.batch import BatchSolver, default_batch_params
     lps = [lpmod.config3_instance(k) for k in range(count)]
+    log("batched: %d LPs generated; tuning" % count)
     with BatchSolver(lps) as B:
         best, tune_ms, tried = B.tune(sample=tune_sample, base=default_batch_params(max_iterations=100000))
+        log("batched: tuned (%d candidates); solving" % tried)
         best.use_graph = 1
         first_ms = B.solve(best)
         times = [B.replay() for _ in range(reps)]
@@ -98,6 +108,7 @@ def batched_family(count, tune_sample, reps, cpu_sample):
     except Exception:
         pass
     if cpu_sample > 0:
+        log("batched: CPU oracle on %d instances" % cpu_sample)
         import multiprocessing as mp
         cores = min(16, os.cpu_count() or 1)
         ks = list(range(cpu_sample))
@@ -109,6 +120,108 @@ def batched_family(count, tune_sample, reps, cpu_sample):
         out["cpu_baseline"] = {"value": round(cpu_sample / wall, 2), "unit": "solves/s", "cores": cores,
                                "kind": "port", "single_thread_solves_per_s": round(cpu_sample / single, 2),
                                "sample": "oracle on instances 0..%d of the same family, one process per core"
+                                         % (cpu_sample - 1)}
+    return out
+
+
+def _oracle_family_solve(args):
+    import time as _t
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as ora
+    from parallel_ds_amd import lp as lpmod
+    m, n, k, seed, sigma, s = args
+    lp = lpmod.perturbed_instance(lpmod.gen_kcol(m, n, k, seed), sigma, s)
+    t0 = _t.perf_counter()
+    r, _, _ = ora.solve(lp)
+    return _t.perf_counter() - t0, r.status, r.Z
+
+
+def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, seed=4, sigma=0.1):
+    """Learned warm start vs slack start on the same GPU: an LP family of the single-LP recipe
+    (k nonzeros per column) whose b and c are perturbed by lognormal(0, sigma); the network is
+    trained on the bases the batched GPU solver finds for n_train instances and evaluated on
+    n_eval held-out instances, all solved in one batch per start."""
+    import numpy as np
+    from parallel_ds_amd import lp as lpmod
+    from parallel_ds_amd.batch import BatchSolver, default_batch_params
+    from parallel_ds_amd.warm import Mlp
+    base = lpmod.gen_kcol(m, n, k, seed)
+    train = [lpmod.perturbed_instance(base, sigma, 10000 + s) for s in range(n_train)]
+    held = [lpmod.perturbed_instance(base, sigma, 20000 + s) for s in range(n_eval)]
+    out = {"workload": "LP family m=%d n=%d, %d nnz/col (seed %d), b and c x lognormal(0, %g); train on %d "
+                       "GPU-solved instances (seeds 10000+), evaluate on %d held-out (seeds 20000+)"
+                       % (m, n, k, seed, sigma, n_train, n_eval)}
+    log("learned warm start: %d + %d instances generated; tuning on the held-out batch" % (n_train, n_eval))
+    with BatchSolver(held) as E:
+        best, _, _ = E.tune(sample=min(256, n_eval), base=default_batch_params(max_iterations=200000))
+        log("learned warm start: slack-start solves")
+        prm = best
+        prm.use_graph = 1
+        # slack start
+        E.set_warm(None)
+        E.solve(prm)
+        t_slack = [E.replay() for _ in range(reps)]
+        st_s, obj_s, it_s, _, _ = E.results()
+        with BatchSolver(train) as T, Mlp(seed=1) as net:
+            log("learned warm start: solving the training instances")
+            T.solve(prm)
+            st_t, _, it_t, _, _ = T.results()
+            log("learned warm start: training %d epochs" % epochs)
+            losses, train_ms = net.fit(T, epochs=epochs, minibatch=8192, lr=1e-3, seed=3)
+            log("learned warm start: losses %s; predicting" % [round(float(v), 4) for v in losses])
+            tr_ms, tr_flops = net.train_stats()
+            # learned warm start: features -> inference -> top-m -> repaired starting bases
+            pred = [E.warm_start(net) for _ in range(reps)]
+        E.solve(prm)
+        t_warm = [E.replay() for _ in range(reps)]
+        st_w, obj_w, it_w, _, _ = E.results()
+        rep = E.repairs()
+    ms_s = float(np.median(t_slack))
+    ms_pred = float(np.median([p[0] for p in pred]))
+    ms_inf = float(np.median([p[1] for p in pred]))
+    ms_w = float(np.median(t_warm)) + ms_pred
+    ok = (st_w == st_s) & (st_s == 0)
+    rel = np.abs(obj_w - obj_s) / np.maximum(1.0, np.abs(obj_s))
+    # inference kernel: 2 flops per MAC over the three hidden layers and the output
+    rows = (m + n) * n_eval
+    inf_flops = 2.0 * rows * (32 * 256 + 2 * 256 * 256 + 256)
+    out.update({
+        "slack": {"solves_per_s": round(n_eval / (ms_s * 1e-3), 2), "ms_per_batch": round(ms_s, 3),
+                  "mean_iterations": round(float(it_s.mean()), 1),
+                  "us_per_lp_iteration": round(1e3 * ms_s / max(1.0, float(it_s.sum())), 4)},
+        "learned_warm": {"solves_per_s": round(n_eval / (ms_w * 1e-3), 2), "ms_per_batch": round(ms_w, 3),
+                         "predict_ms": round(ms_pred, 3), "solve_ms": round(ms_w - ms_pred, 3),
+                         "mean_iterations": round(float(it_w.mean()), 1),
+                         "us_per_lp_iteration": round(1e3 * (ms_w - ms_pred) / max(1.0, float(it_w.sum())), 4),
+                         "repaired_columns_mean": round(float(rep.mean()), 2)},
+        "speedup_solves_per_s": round(ms_s / ms_w, 3),
+        "iteration_ratio": round(float(it_s.mean()) / max(1e-9, float(it_w.mean())), 3),
+        "status_match_and_optimal": int(ok.sum()), "of": n_eval,
+        "max_obj_rel_err_warm_vs_slack": float(rel.max()),
+        "training": {"instances_optimal": int((st_t == 0).sum()), "mean_iterations": round(float(it_t.mean()), 1),
+                     "epochs": epochs, "epoch_loss": [round(float(v), 4) for v in losses],
+                     "train_ms": round(train_ms, 1),
+                     "train_gemm_tflops": round(tr_flops / (tr_ms * 1e-3) / 1e12, 2) if tr_ms > 0 else None},
+        "inference_roofline": {"kernel": "k_infer (fused 3-layer BF16 MFMA)", "bound": "mfma",
+                               "achieved": round(inf_flops / (ms_inf * 1e-3) / 1e12, 2), "peak": 2500.0,
+                               "unit": "TFLOP/s", "frac": round(inf_flops / (ms_inf * 1e-3) / 1e12 / 2500.0, 4),
+                               "avg_launch_us": round(ms_inf * 1e3, 2), "rows": rows},
+        "tuned": {"block": prm.block, "lds_inv_max_m": prm.lds_inv_max_m, "wgs_per_cu": prm.wgs_per_cu},
+    })
+    if cpu_sample > 0:
+        log("learned warm start: CPU oracle on %d instances" % cpu_sample)
+        import multiprocessing as mp
+        cores = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        with mp.Pool(cores) as pool:
+            res = pool.map(_oracle_family_solve, [(m, n, k, seed, sigma, 20000 + s) for s in range(cpu_sample)])
+        wall = time.perf_counter() - t0
+        zs = np.array([r[2] for r in res])
+        out["cpu_baseline"] = {"value": round(cpu_sample / wall, 2), "unit": "solves/s", "cores": cores,
+                               "kind": "port", "single_thread_solves_per_s": round(cpu_sample / sum(r[0] for r in res), 2),
+                               "max_obj_rel_err_gpu_slack_vs_oracle":
+                                   float(np.max(np.abs(obj_s[:cpu_sample] - zs) / np.maximum(1.0, np.abs(zs)))),
+                               "sample": "oracle (slack start) on held-out instances 0..%d, one process per core"
                                          % (cpu_sample - 1)}
     return out
 
@@ -129,6 +242,11 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--batch-cpu-sample", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--warm-train", type=int, default=10000)
+    ap.add_argument("--warm-eval", type=int, default=1000)
+    ap.add_argument("--warm-epochs", type=int, default=8)
+    ap.add_argument("--warm-cpu-sample", type=int, default=64)
+    ap.add_argument("--no-warm", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -140,6 +258,7 @@ def main():
     from parallel_ds_amd import lp as lpmod
 
     lp = lpmod.config1(seed=args.seed, m=args.m, n=args.n, k=args.k)
+    log("single LP m=%d n=%d: %d warmup + %d timed iterations" % (lp.m, lp.n, args.warmup, args.steps))
     prm = default_params(refactor_freq=args.refactor, chunk=args.chunk,
                          max_iterations=args.warmup + args.steps,
                          mark_iteration=args.warmup, time_kernels=1)
@@ -187,10 +306,14 @@ def main():
                      "launches": r.spmv_launches},
     }
     if not args.no_cpu_baseline:
+        log("single LP: CPU oracle over the same window")
         out["cpu_baseline"] = cpu_baseline(lp, args.refactor, args.warmup, timed)
     if not args.no_batch:
         out["batched"] = batched_family(args.batch_count, 256, args.batch_reps,
                                         0 if args.no_cpu_baseline else args.batch_cpu_sample)
+    if not args.no_warm:
+        out["learned_warm_start"] = learned_warm(args.warm_train, args.warm_eval, args.warm_epochs, args.batch_reps,
+                                                 0 if args.no_cpu_baseline else args.warm_cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -145,6 +145,10 @@ typedef struct {
   int lds_inv_max_m;      /* LPs with m <= this keep B^{-1} in LDS */
   int wgs_per_cu;         /* resident workgroups per CU of the persistent grid */
   int use_graph;          /* capture the launch in a hipGraph */
+  int cleanup;            /* 1 (default): after a cost perturbation, if the original costs leave
+                             dual infeasibilities, resume the dual simplex from the final basis.
+                             0: the reference's behaviour (its primal correction is a stub,
+                             sequentialDualSimplex.cpp:113-116, so that basis is returned) */
 } ds_batch_params;
 
 void ds_batch_default_params(ds_batch_params* p);

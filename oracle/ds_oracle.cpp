@@ -907,6 +907,21 @@ struct Solver {
     if (perturbed) {
       c = original_costs;
       dualSimplexInit();
+      /* postProcess (tpp:123-159) calls callPrimalSolver when the restored
+       * costs leave dual infeasibilities; that correction is a stub in the
+       * reference (sequentialDualSimplex.cpp:113-116).  cleanup = 1 resumes
+       * the dual simplex from the final basis instead. */
+      if (P.cleanup && ok && !checkDualFeasible()) {
+        int st2 = phase1();
+        if (st2 == 1) {
+          status = ORA_DUAL_INFEASIBLE;
+          solved = false;
+          Z = 0.0;
+          return;
+        }
+        setPrimalVars();
+        ok = phase2();
+      }
     }
     (void)ok;
     Z = obj;

@@ -39,6 +39,9 @@ typedef struct {
   int perturb;             /* 1 = reference rules (presolve check + cycling), 0 = never perturb */
   uint64_t perturb_seed;
   int verbose;
+  int cleanup;             /* 0 = reference (stub primal correction after perturbation);
+                              1 = resume the dual simplex when the original costs leave dual
+                              infeasibilities */
 } ora_params;
 
 typedef struct {

@@ -32,7 +32,7 @@ class Result(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("perturb", C.c_int),
                 ("perturb_seed", C.c_uint64), ("block", C.c_int), ("lds_inv_max_m", C.c_int),
-                ("wgs_per_cu", C.c_int), ("use_graph", C.c_int)]
+                ("wgs_per_cu", C.c_int), ("use_graph", C.c_int), ("cleanup", C.c_int)]
 
 
 class DsError(RuntimeError):

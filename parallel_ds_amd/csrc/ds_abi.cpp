@@ -273,6 +273,7 @@ void ds_batch_default_params(ds_batch_params* p) {
   p->lds_inv_max_m = 64;
   p->wgs_per_cu = 2;
   p->use_graph = 1;
+  p->cleanup = 1;
 }
 
 int ds_batch_load(ds_ctx* ctx, int count, const int* m, const int* n, const long long* nnz_off, const int* col_ptr,

@@ -1138,9 +1138,25 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
         S.set_primal();
         status = S.phase2();
         if (S.sh->perturbed && (status == 0 || status == 1)) {
+          // postProcess (tpp:123-159): original costs, primal values and duals
+          // recomputed.  The reference then calls a primal correction that is
+          // a stub (sequentialDualSimplex.cpp:113-116), leaving a basis that
+          // may be dual infeasible for the original costs.  With `cleanup` the
+          // dual simplex resumes from that basis instead (the perturbation
+          // flag stays set, so no second perturbation).
           for (int j = threadIdx.x; j < N; j += BS) S.c[j] = S.corig[j];
           __syncthreads();
           S.dual_init();
+          if (D.cleanup && status == 0 && !S.dual_feasible()) {
+            const int st2 = S.phase1();
+            if (st2 == 1) status = 2;
+            else if (st2 == 2) status = 3;
+            else if (st2 == 3) status = 5;
+            else {
+              S.set_primal();
+              status = S.phase2();
+            }
+          }
         }
       }
     }

@@ -43,6 +43,7 @@ struct BatchDev {
   int refactor_freq;
   int max_iter;
   int perturb;
+  int cleanup;               // resume the dual simplex if the restored costs leave dual infeasibilities
   unsigned long long seed;
 };
 

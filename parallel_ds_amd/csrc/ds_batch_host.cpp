@@ -253,6 +253,7 @@ BatchDev BatchSolver::make_dev() const {
   D.refactor_freq = 150;
   D.max_iter = 0;
   D.perturb = 0;
+  D.cleanup = 0;
   D.seed = 0;
   return D;
 }
@@ -270,6 +271,7 @@ void BatchSolver::enqueue(const ds_batch_params& p) {
     D.refactor_freq = p.refactor_freq > 0 ? p.refactor_freq : 150;
     D.max_iter = p.max_iterations;
     D.perturb = p.perturb;
+    D.cleanup = p.cleanup;
     D.seed = p.perturb_seed;
     launch_batch_reset(st_, D.next);
     launch_batch(st_, D, C.grid, p.block);

@@ -12,7 +12,8 @@ ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 class OraParams(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
-                ("tie_rule", C.c_int), ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("verbose", C.c_int)]
+                ("tie_rule", C.c_int), ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("verbose", C.c_int),
+                ("cleanup", C.c_int)]
 
 
 class OraResult(C.Structure):
@@ -52,8 +53,9 @@ def P(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def params(refactor_freq=150, max_iterations=0, ratio_test=0, tie_rule=0, perturb=1, seed=0x5eed, verbose=0):
-    return OraParams(refactor_freq, max_iterations, ratio_test, tie_rule, perturb, seed, verbose)
+def params(refactor_freq=150, max_iterations=0, ratio_test=0, tie_rule=0, perturb=1, seed=0x5eed, verbose=0,
+           cleanup=0):
+    return OraParams(refactor_freq, max_iterations, ratio_test, tie_rule, perturb, seed, verbose, cleanup)
 
 
 def solve(lp, **kw):

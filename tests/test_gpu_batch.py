@@ -94,3 +94,20 @@ def test_batch_rejects_invalid_warm_basis(gpu, family):
         bad = [np.zeros(L.m, np.int32) for L in sub]  # repeated ids
         with pytest.raises(Exception):
             B.set_warm(bad)
+
+
+def test_batch_cleanup_after_perturbation_matches_highs(gpu):
+    """Learned-warm-start family instances against HiGHS.  Instance 20443 is perturbed on its
+    slack-start path; with the default cleanup its objective still matches HiGHS to 1e-9."""
+    gold = {g["seed"]: g["objective"] for g in HI["warm_family"]}
+    base = lpmod.gen_kcol(256, 640, 5, 4)
+    seeds = list(range(20000, 20016)) + [20443]
+    lps = [lpmod.perturbed_instance(base, 0.1, s) for s in seeds]
+    with BatchSolver(lps) as B:
+        B.solve(default_batch_params(max_iterations=200000))
+        st, obj, _, _, _ = B.results()
+        B.solve(default_batch_params(max_iterations=200000, cleanup=0))
+        st0, obj0, _, _, _ = B.results()
+    for k, s in enumerate(seeds):
+        assert st[k] == 0 and abs(obj[k] - gold[s]) <= 1e-9 * abs(gold[s]), (s, obj[k], gold[s])
+        assert st0[k] == 0 and abs(obj0[k] - gold[s]) <= 1e-7 * abs(gold[s]), (s, obj0[k], gold[s])

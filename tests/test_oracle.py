@@ -193,3 +193,20 @@ def test_perturbation_is_seeded_and_recorded():
     r1, x1, _ = ora.solve(L, seed=1)
     r2, x2, _ = ora.solve(L, seed=1)
     assert r1.perturbed == 1 and r1.iterations == r2.iterations and r1.Z == r2.Z
+
+
+def test_cleanup_after_perturbation_reaches_the_optimum():
+    """Instance 20443 of the learned-warm-start family: its slack-start solve is perturbed and,
+    with the reference's stub primal correction, ends dual infeasible for the original costs
+    (8.7e-9 above HiGHS); resuming the dual simplex (cleanup=1) reaches HiGHS' optimum."""
+    import json
+    from parallel_ds_amd import lp as lpmod
+    gold = {g["seed"]: g["objective"] for g in
+            json.load(open(os.path.join(G, "highs_objectives.json")))["warm_family"]}
+    base = lpmod.gen_kcol(256, 640, 5, 4)
+    for s in list(range(20000, 20012)) + [20443]:
+        L = lpmod.perturbed_instance(base, 0.1, s)
+        r, _, _ = ora.solve(L, cleanup=1)
+        assert r.status == 0 and abs(r.Z - gold[s]) <= 1e-9 * abs(gold[s]), (s, r.Z, gold[s])
+    r0, _, _ = ora.solve(lpmod.perturbed_instance(base, 0.1, 20443), cleanup=0)
+    assert r0.perturbed and abs(r0.Z - gold[20443]) > 1e-9 * abs(gold[20443])
