@@ -99,6 +99,7 @@ def batched_family(count, tune_sample, reps, cpu_sample):
            "mean_iterations": round(float(it.mean()), 1),
            "ms_per_iter_per_lp": round(ms / max(1, float(it.sum())) * count, 5),
            "tuned": {"block": best.block, "lds_inv_max_m": best.lds_inv_max_m, "wgs_per_cu": best.wgs_per_cu,
+                     "refactor_freq": best.refactor_freq,
                      "candidates": tried, "sample": tune_sample, "sample_ms": round(tune_ms, 3)}}
     try:
         import json as _j
@@ -206,7 +207,8 @@ def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, s
                                "achieved": round(inf_flops / (ms_inf * 1e-3) / 1e12, 2), "peak": 2500.0,
                                "unit": "TFLOP/s", "frac": round(inf_flops / (ms_inf * 1e-3) / 1e12 / 2500.0, 4),
                                "avg_launch_us": round(ms_inf * 1e3, 2), "rows": rows},
-        "tuned": {"block": prm.block, "lds_inv_max_m": prm.lds_inv_max_m, "wgs_per_cu": prm.wgs_per_cu},
+        "tuned": {"block": prm.block, "lds_inv_max_m": prm.lds_inv_max_m, "wgs_per_cu": prm.wgs_per_cu,
+                  "refactor_freq": prm.refactor_freq},
     })
     if cpu_sample > 0:
         log("learned warm start: CPU oracle on %d instances" % cpu_sample)

@@ -161,7 +161,9 @@ int ds_batch_replay(ds_ctx* ctx, double* ms);  /* relaunch the captured graph */
 int ds_batch_results(ds_ctx* ctx, int* status, double* obj, int* iters, int* basis /* sum m */,
                      double* x /* sum (n+m) */);
 /* Autotuner: times workgroup size x LDS threshold x occupancy on a sample of
- * `sample` LPs; writes the fastest into *best (its other fields are kept). */
+ * `sample` LPs, then the refactor interval (150, 300, 600) at the best shape;
+ * every candidate must reproduce the statuses and objectives (1e-9) of the
+ * first.  Writes the fastest into *best (its other fields are kept). */
 int ds_batch_tune(ds_ctx* ctx, int sample, ds_batch_params* best, double* best_ms, int* tried);
 
 /* ---- Learned warm start ----

@@ -286,6 +286,7 @@ struct Solver {
   std::vector<Eta> etas;
   bool perturbed = false;
   bool warm = false;
+  int phase1_iterations = 0;
   int iteration = 0;
   double obj = 0.0;
   int status = ORA_OPTIMAL;
@@ -896,6 +897,7 @@ struct Solver {
     if (P.perturb && checkPerturbNeed()) perturbCosts();
     initBetaWeights(!warm);  /* exact weights for a given starting basis */
     int st = phase1();
+    phase1_iterations = iteration;
     if (st == 1) {
       status = ORA_DUAL_INFEASIBLE;
       solved = false;
@@ -964,6 +966,7 @@ extern "C" int ora_solve_from(int m, int n, const int* col_ptr, const int* row_i
     result->refactors = S.refactors;
     result->phase1_restarts = S.phase1_restarts;
     result->Z = Z;
+    result->phase1_iterations = S.phase1_iterations;
     if (x_out)
       for (int j = 0; j < M.N; ++j) x_out[j] = S.x[j];
     if (basis_out)

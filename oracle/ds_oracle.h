@@ -52,6 +52,7 @@ typedef struct {
   int refactors;
   int phase1_restarts;
   double Z;                /* objective with the offset removed */
+  int phase1_iterations;   /* iterations of the initial phase 1 */
 } ora_result;
 
 /* Whole solve.  A is m x n (structurals only) in CSC.  Row types/ranges follow

@@ -34,6 +34,14 @@ __device__ inline double dev_uniform(unsigned long long& s) {
 // plain pointers the compiler must assume a store to B may change the LDS
 // vectors and serialises every element behind the previous store.
 
+// Orders the LDS (and global) accesses of one wavefront: every lane's
+// earlier stores are visible to every lane's later loads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row of flat index e < m*m (exact for m <= 4096: e/m is at least 0.5/m away
 // from the next integer, far above the float rounding error).
 __device__ __forceinline__ int flat_row(int e, float rinv, float half) {
@@ -274,6 +282,10 @@ struct BShared {
   int sing;
   int nblocked;
   int nrep;
+  int pn, pnext, nperm;
+  int pl[GJ_NB_MAX];
+  int prow_of[2 * GJ_NB_MAX], psrc[2 * GJ_NB_MAX];
+  double prp[GJ_NB_MAX];
   int blocked[64];
   double delta, theta, obj, f_p, corr, tmp;
   unsigned long long rng;
@@ -318,8 +330,13 @@ struct LpSolve {
   int* rpos;
   int* rflag;
   int* rat;
+  double* Pn;          // panel columns (m x gj_nb), LDS
+  double* Vb;          // row-operation vectors of the panel (m x gj_nb)
+  double* Rb;          // deferred multipliers (gj_nb x m)
+  int gj_nb;           // panel width of the factorization
   int repair_on;       // slack substitution for (near-)singular columns
   double repair_tol;
+  double sing_tol;     // a pivot at or below it is singular
   double* Binv;
   BShared* sh;
   int K, max_iter, perturb_on;
@@ -377,6 +394,7 @@ struct LpSolve {
         }
       }
       for (int s = 0; s < m; ++s) rflag[s] = basis[s] >= n ? 1 : 0;
+      sh->sing = 0;
     }
     for (int e = tid; e < mm; e += BS) Binv[e] = 0.0;
     sync();
@@ -389,63 +407,207 @@ struct LpSolve {
       }
     }
     sync();
-    for (int s = 0; s < m; ++s) {
-      if (basis[s] >= n) continue;
-      double bv = -1.0;
-      int bi = INT_MAX;
-      for (int r = tid; r < m; r += BS)
-        if (!rflag[r]) {
-          const double a = fabs(Binv[r * m + s]);
-          if (a >= 0.0) argmax_merge(bv, bi, a, r);
-        }
-      block_argmax<BS>(bv, bi);
-      if (repair_on && !(bi != INT_MAX && bv > repair_tol)) {
-        // Slack substitution: column s depends (numerically) on the columns
-        // already pivoted.  Every position below s is used (slack positions by
-        // the slack pass, structural ones by their steps), so position s is the
-        // lowest free one; the slack of the original row held there enters.
-        // Transformed by the steps so far, that slack's column is the unit
-        // vector at s: row operations only add multiples of pivot rows, and
-        // the row at s has not been one.
-        if (tid == 0) {
-          const int jold = basis[s], jnew = n + rat[s];
-          const int k = posN[jnew];
-          nonbasis[k] = jold;
-          posN[jold] = k;
-          posN[jnew] = -1;
-          basis[s] = jnew;
-          sh->nrep += 1;
-        }
-        for (int i = tid; i < m; i += BS) Binv[i * m + s] = (i == s) ? 1.0 : 0.0;
-        bi = s;
-        bv = 1.0;
-        sync();
-      }
-      if (bi == INT_MAX || !(bv > 0.0)) {
-        if (tid == 0) sh->sing = 1;
-        sync();
-        return false;
-      }
+    // Structural columns in panels of up to gj_nb positions.  Inside a panel
+    // the steps run on an LDS copy of the panel's columns (Pn, m x nb); row
+    // exchanges are applied at once to the stored matrix; the elimination of
+    // every other column is deferred: step t is the row operation
+    // E_t = I + v_t e_{s_t}^T (v_t[i] = -w_t[i]/pv_t, v_t[s_t] = 1/pv_t - 1),
+    // and a later exchange of rows a, b turns it into I + (P v_t) e_{s_t}^T
+    // (s_t is used, a and b are not), so the stored v's are exchanged too.  At
+    // the end of the panel each other column g becomes g + V r with
+    // r_t = g[s_t] + sum_{t'<t} v_t'[s_t] r_t': one pass over the matrix per
+    // panel instead of one per step.  cmap serves as the panel-index map of a
+    // position while the factorization runs (it is rebuilt at the end).
+    const int NB = gj_nb;
+    for (int c = tid; c < m; c += BS) cmap[c] = -1;
+    if (tid == 0) sh->pnext = 0;
+    sync();
+    while (true) {
       if (tid == 0) {
-        piv[s] = bi;
-        rflag[s] = 1;
-        const int t = rat[s];
-        rat[s] = rat[bi];
-        rat[bi] = t;
+        int cnt = 0, pos = sh->pnext;
+        for (; pos < m && cnt < NB; ++pos)
+          if (basis[pos] < n) {
+            sh->pl[cnt] = pos;
+            cmap[pos] = cnt;
+            ++cnt;
+          }
+        sh->pn = cnt;
+        sh->pnext = pos;
+        sh->nperm = 0;
       }
-      // Swap rows s and bi while reading the pivot row into prow.  Column s
-      // of row bi is left alone here (the pivot value is read from it by every
-      // thread); the elimination below overwrites it.
-      const double pv = Binv[bi * m + s];
-      for (int cc = tid; cc < m; cc += BS) {
-        double a = Binv[s * m + cc];
-        double b = Binv[bi * m + cc];
-        if (bi != s && cc != s) Binv[bi * m + cc] = a;
-        prow[cc] = ((cc == s) ? 1.0 : b) / pv;
-      }
-      for (int i = tid; i < m; i += BS) w[i] = (i == s) ? 0.0 : ((i == bi) ? Binv[s * m + s] : Binv[i * m + s]);
       sync();
-      inv_eliminate<BS>(Binv, w, prow, m, s, s, tid);
+      const int nb = sh->pn;
+      if (nb == 0) break;
+      for (int e = tid; e < m * nb; e += BS) {
+        const int i = e / nb, j = e - i * nb;
+        Pn[e] = Binv[i * m + sh->pl[j]];
+      }
+      sync();
+      // The nb steps of the panel run in wavefront 0 alone (the panel is
+      // m x nb values); the other wavefronts wait at the barrier after it.
+      if (tid < WAVE) {
+        const int lane = tid;
+        for (int j = 0; j < nb; ++j) {
+          const int sj = sh->pl[j];
+          double bv = -1.0;
+          int bi = INT_MAX;
+          for (int r = lane; r < m; r += WAVE)
+            if (!rflag[r]) {
+              const double a = fabs(Pn[r * nb + j]);
+              if (a >= 0.0) argmax_merge(bv, bi, a, r);
+            }
+          wave_argmax(bv, bi);
+          if (repair_on && !(bi != INT_MAX && bv > repair_tol)) {
+            // Slack substitution: column sj depends (numerically) on the
+            // columns already pivoted.  Every position below sj is used (slack
+            // positions by the slack pass, structural ones by their steps), so
+            // position sj is the lowest free one; the slack of the original row
+            // held there enters.  Transformed by the steps so far, that slack's
+            // column is the unit vector at sj: row operations only add
+            // multiples of pivot rows, and the row at sj has not been one.
+            if (lane == 0) {
+              const int jold = basis[sj], jnew = n + rat[sj];
+              const int k = posN[jnew];
+              nonbasis[k] = jold;
+              posN[jold] = k;
+              posN[jnew] = -1;
+              basis[sj] = jnew;
+              sh->nrep += 1;
+            }
+            for (int i = lane; i < m; i += WAVE) Pn[i * nb + j] = (i == sj) ? 1.0 : 0.0;
+            bi = sj;
+            bv = 1.0;
+            wave_lds_sync();
+          }
+          if (bi == INT_MAX || !(bv > sing_tol)) {
+            if (lane == 0) sh->sing = 1;
+            break;
+          }
+          if (lane == 0) {
+            piv[sj] = bi;
+            rflag[sj] = 1;
+            const int t = rat[sj];
+            rat[sj] = rat[bi];
+            rat[bi] = t;
+          }
+          if (bi != sj) {
+            // The exchange of rows sj and bi in the columns outside the panel
+            // is recorded (row -> source row) and applied once per panel.
+            if (lane == 0) {
+              int ka = -1, kb = -1;
+              for (int k = 0; k < sh->nperm; ++k) {
+                if (sh->prow_of[k] == sj) ka = k;
+                if (sh->prow_of[k] == bi) kb = k;
+              }
+              if (ka < 0) {
+                ka = sh->nperm++;
+                sh->prow_of[ka] = sj;
+                sh->psrc[ka] = sj;
+              }
+              if (kb < 0) {
+                kb = sh->nperm++;
+                sh->prow_of[kb] = bi;
+                sh->psrc[kb] = bi;
+              }
+              const int t = sh->psrc[ka];
+              sh->psrc[ka] = sh->psrc[kb];
+              sh->psrc[kb] = t;
+            }
+            for (int j2 = lane; j2 < nb; j2 += WAVE) {
+              const double a = Pn[sj * nb + j2];
+              Pn[sj * nb + j2] = Pn[bi * nb + j2];
+              Pn[bi * nb + j2] = a;
+            }
+            for (int t = lane; t < j; t += WAVE) {
+              const double a = Vb[sj * nb + t];
+              Vb[sj * nb + t] = Vb[bi * nb + t];
+              Vb[bi * nb + t] = a;
+            }
+          }
+          wave_lds_sync();
+          const double pv = Pn[sj * nb + j];
+          const double ipv = 1.0 / pv;
+          for (int i = lane; i < m; i += WAVE) Vb[i * nb + j] = (i == sj) ? ipv - 1.0 : -Pn[i * nb + j] / pv;
+          for (int j2 = lane; j2 < nb; j2 += WAVE) sh->prp[j2] = Pn[sj * nb + j2];
+          wave_lds_sync();
+          for (int i = lane; i < m; i += WAVE) {
+            const double vi = Vb[i * nb + j];
+            for (int j2 = 0; j2 < nb; ++j2) {
+              const int e = i * nb + j2;
+              if (j2 == j) Pn[e] = (i == sj) ? ipv : vi;
+              else Pn[e] = Pn[e] + vi * sh->prp[j2];
+            }
+          }
+          wave_lds_sync();
+        }
+      }
+      sync();
+      if (sh->sing) return false;
+      // Row exchanges of the panel on the columns outside it: each thread
+      // owns columns, reads every source row first, then writes.
+      {
+        const int np = sh->nperm;
+        if (np > 0) {
+          for (int c = tid; c < m; c += BS) {
+            if (cmap[c] >= 0) continue;
+            double v[2 * GJ_NB_MAX];
+#pragma unroll
+            for (int k = 0; k < 2 * GJ_NB_MAX; ++k)
+              if (k < np) v[k] = Binv[sh->psrc[k] * m + c];
+#pragma unroll
+            for (int k = 0; k < 2 * GJ_NB_MAX; ++k)
+              if (k < np) Binv[sh->prow_of[k] * m + c] = v[k];
+          }
+          sync();
+        }
+      }
+      // Deferred elimination of the columns outside the panel.
+      for (int c = tid; c < m; c += BS) {
+        if (cmap[c] >= 0) continue;
+        double r[GJ_NB_MAX];
+#pragma unroll
+        for (int t = 0; t < GJ_NB_MAX; ++t) {
+          if (t >= nb) break;
+          double v = Binv[sh->pl[t] * m + c];
+          for (int t2 = 0; t2 < t; ++t2) v += Vb[sh->pl[t] * nb + t2] * r[t2];
+          r[t] = v;
+          Rb[t * m + c] = v;
+        }
+      }
+      sync();
+      {
+        // G += V R over the columns outside the panel, the panel columns
+        // copied back; four elements in flight per thread.
+        const float rinv = 1.0f / (float)m, half = 0.5f * rinv;
+        for (int e0 = tid; e0 < mm; e0 += 4 * BS) {
+          double g[4];
+          int ii[4], cc[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int e = e0 + k * BS;
+            const int i = flat_row(e, rinv, half);
+            ii[k] = i;
+            cc[k] = e - i * m;
+            g[k] = (e < mm && cmap[cc[k]] < 0) ? Binv[e] : 0.0;
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int e = e0 + k * BS;
+            if (e >= mm) continue;
+            const int pj = cmap[cc[k]];
+            double v;
+            if (pj >= 0) {
+              v = Pn[ii[k] * nb + pj];
+            } else {
+              v = g[k];
+              for (int t = 0; t < nb; ++t) v += Vb[ii[k] * nb + t] * Rb[t * m + cc[k]];
+            }
+            Binv[e] = v;
+          }
+        }
+      }
+      sync();
+      for (int j = tid; j < nb; j += BS) cmap[sh->pl[j]] = -1;
       sync();
     }
     if (tid == 0) {
@@ -1000,16 +1162,24 @@ struct LpSolve {
 
 }  // namespace
 
-size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds) {
+size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds, int gj_nb) {
   const int N = mmax + nmax;
   // c, x, d, wN (N each); beta, f, rho, aq, tau, w, prow (m each); ar (n).
   // The sort buffer of checkPerturbNeed aliases wN..ar.
   size_t dbl = (size_t)4 * N + (size_t)7 * mmax + nmax;
+  // factorization panel: Pn, Vb (m x nb each), Rb (nb x m)
+  dbl += (size_t)3 * gj_nb * mmax;
   // basis (m), nonbasis (n), posN (N), piv, cmap, rpos, rflag, rat (m each).
   size_t ints = (size_t)mmax + nmax + N + 5 * (size_t)mmax;
   size_t bytes = dbl * 8 + ints * 4 + sizeof(BShared) + 64;
   if (inv_in_lds) bytes += (size_t)mmax * mmax * 8;
   return bytes;
+}
+
+int batch_panel_width(int mmax, int nmax, int inv_in_lds, size_t limit) {
+  for (int nb = GJ_NB_MAX; nb >= 1; nb /= 2)
+    if (batch_lds_bytes(mmax, nmax, inv_in_lds, nb) <= limit) return nb;
+  return 0;
 }
 
 template <int BS>
@@ -1042,6 +1212,11 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
   } else {
     S.Binv = D.gscratch + (size_t)blockIdx.x * mmax * mmax;
   }
+  S.gj_nb = D.gj_nb;
+  S.Pn = after;
+  S.Vb = S.Pn + (size_t)D.gj_nb * mmax;
+  S.Rb = S.Vb + (size_t)D.gj_nb * mmax;
+  after = S.Rb + (size_t)D.gj_nb * mmax;
   int* ip = (int*)after;
   S.basis = ip;
   S.nonbasis = S.basis + mmax;
@@ -1122,9 +1297,13 @@ __global__ __launch_bounds__(BS) void k_batch(BatchDev D) {
     int status;
     // A given starting basis may be repaired by slack substitution in its
     // first factorization; its DSE weights are the exact row norms.
+    // Without repair a given start is singular once a pivot is numerically
+    // zero (the blocked elimination does not cancel exactly).
     S.repair_on = (D.warm && D.repair_tol > 0.0) ? 1 : 0;
+    S.sing_tol = (D.warm && !S.repair_on) ? 1e-12 : 0.0;
     bool ok = S.reinvert();
     S.repair_on = 0;
+    S.sing_tol = 0.0;
     if (!ok) {
       status = 5;
     } else {
@@ -1178,7 +1357,7 @@ __global__ void k_batch_reset(int* next) {
 }
 
 void launch_batch(hipStream_t s, const BatchDev& D, int grid, int block) {
-  size_t lds = batch_lds_bytes(D.mmax, D.nmax, D.inv_in_lds);
+  size_t lds = batch_lds_bytes(D.mmax, D.nmax, D.inv_in_lds, D.gj_nb);
   switch (block) {
     case 128: k_batch<128><<<grid, 128, lds, s>>>(D); break;
     case 512: k_batch<512><<<grid, 512, lds, s>>>(D); break;

@@ -4,6 +4,8 @@
 
 namespace pds {
 
+constexpr int GJ_NB_MAX = 8;  // widest panel of the batched factorization
+
 // A batch of LPs  min c^T x, A x <= b, x >= 0  (packing form), each held in
 // CSC and CSR with per-LP offsets into concatenated arrays.  Variables of LP k
 // are its n_k structurals then its m_k slacks (computational form).
@@ -40,6 +42,7 @@ struct BatchDev {
   double* gscratch;          // per-workgroup global inverse (mmax^2) when not in LDS
   int mmax, nmax;
   int inv_in_lds;
+  int gj_nb;                 // factorization panel width (1..GJ_NB_MAX), sized to the LDS left
   int refactor_freq;
   int max_iter;
   int perturb;
@@ -47,7 +50,9 @@ struct BatchDev {
   unsigned long long seed;
 };
 
-size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds);
+size_t batch_lds_bytes(int mmax, int nmax, int inv_in_lds, int gj_nb);
+// Widest panel (GJ_NB_MAX, 4, 2, 1) whose LDS footprint fits `limit` bytes (0 if none).
+int batch_panel_width(int mmax, int nmax, int inv_in_lds, size_t limit);
 void launch_batch(hipStream_t s, const BatchDev& D, int grid, int block);
 void launch_batch_reset(hipStream_t s, int* next);
 

@@ -15,6 +15,8 @@
 namespace pds {
 
 namespace {
+// Dynamic LDS per workgroup: 160 KB per CU, less the static reduction buffers.
+constexpr size_t LDS_BUDGET = 160 * 1024 - 2048;
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -250,6 +252,7 @@ BatchDev BatchSolver::make_dev() const {
   D.mmax = mmax_all_;
   D.nmax = nmax_all_;
   D.inv_in_lds = 0;
+  D.gj_nb = 1;
   D.refactor_freq = 150;
   D.max_iter = 0;
   D.perturb = 0;
@@ -268,6 +271,7 @@ void BatchSolver::enqueue(const ds_batch_params& p) {
     D.mmax = C.mmax;
     D.nmax = C.nmax;
     D.inv_in_lds = C.inv_in_lds ? 1 : 0;
+    D.gj_nb = C.gj_nb;
     D.refactor_freq = p.refactor_freq > 0 ? p.refactor_freq : 150;
     D.max_iter = p.max_iterations;
     D.perturb = p.perturb;
@@ -302,9 +306,9 @@ void BatchSolver::select_warm(const float* z_dev) {
 
 double BatchSolver::run(const std::vector<int>& items, const ds_batch_params& p, bool use_graph) {
   plan(items, p);
-  for (const Class& C : classes_) {
-    size_t lds = batch_lds_bytes(C.mmax, C.nmax, C.inv_in_lds);
-    if (lds > 160 * 1024) throw std::invalid_argument("batch size class exceeds the LDS budget");
+  for (Class& C : classes_) {
+    C.gj_nb = batch_panel_width(C.mmax, C.nmax, C.inv_in_lds, LDS_BUDGET);
+    if (C.gj_nb == 0) throw std::invalid_argument("batch size class exceeds the LDS budget");
   }
   if (use_graph) {
     drop_graph();
@@ -383,12 +387,42 @@ void BatchSolver::tune(int sample, ds_batch_params& best, double& best_ms, int& 
   const int blocks[3] = {128, 256, 512};
   const int ldsm[3] = {0, 96, 128};
   const int occ[3] = {1, 2, 4};
+  const int refac[3] = {150, 300, 600};
   std::vector<int> ref_st(count_);
   std::vector<double> ref_obj(count_);
   bool have_ref = false;
   best_ms = 1e300;
   tried = 0;
-  ds_batch_params base = best;
+  // Every candidate must end each sampled LP with the same status and an
+  // objective within 1e-9 of the first candidate's.
+  auto trial = [&](const ds_batch_params& p) {
+    double ms;
+    try {
+      ms = run(items, p, false);
+    } catch (const std::invalid_argument&) {
+      return;
+    }
+    ++tried;
+    std::vector<int> st(count_);
+    std::vector<double> ob(count_);
+    results(st.data(), ob.data(), nullptr, nullptr, nullptr);
+    if (!have_ref) {
+      ref_st = st;
+      ref_obj = ob;
+      have_ref = true;
+    } else {
+      for (int k : items)
+        if (st[k] != ref_st[k] || std::fabs(ob[k] - ref_obj[k]) > 1e-9 * std::max(1.0, std::fabs(ref_obj[k])))
+          throw std::runtime_error("tuned launch shapes disagree on a result");
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+  };
+  // Stage 1: launch shape (workgroup size x LDS-inverse threshold x
+  // occupancy) at the given refactor interval; stage 2: refactor interval.
+  const ds_batch_params base = best;
   for (int bi = 0; bi < 3; ++bi)
     for (int li = 0; li < 3; ++li)
       for (int oi = 0; oi < 3; ++oi) {
@@ -396,30 +430,15 @@ void BatchSolver::tune(int sample, ds_batch_params& best, double& best_ms, int& 
         p.block = blocks[bi];
         p.lds_inv_max_m = ldsm[li];
         p.wgs_per_cu = occ[oi];
-        double ms;
-        try {
-          ms = run(items, p, false);
-        } catch (const std::invalid_argument&) {
-          continue;
-        }
-        ++tried;
-        std::vector<int> st(count_);
-        std::vector<double> ob(count_);
-        results(st.data(), ob.data(), nullptr, nullptr, nullptr);
-        if (!have_ref) {
-          ref_st = st;
-          ref_obj = ob;
-          have_ref = true;
-        } else {
-          for (int k : items)
-            if (st[k] != ref_st[k] || std::fabs(ob[k] - ref_obj[k]) > 1e-9 * std::max(1.0, std::fabs(ref_obj[k])))
-              throw std::runtime_error("tuned launch shapes disagree on a result");
-        }
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = p;
-        }
+        trial(p);
       }
+  const ds_batch_params shape = best;
+  for (int ri = 0; ri < 3; ++ri) {
+    if (refac[ri] == shape.refactor_freq) continue;
+    ds_batch_params p = shape;
+    p.refactor_freq = refac[ri];
+    trial(p);
+  }
 }
 
 }  // namespace pds

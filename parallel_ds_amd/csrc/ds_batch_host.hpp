@@ -41,7 +41,7 @@ class BatchSolver {
 
  private:
   struct Class {
-    int first, count, mmax, nmax, grid, counter;
+    int first, count, mmax, nmax, grid, counter, gj_nb = 1;
     bool inv_in_lds;
   };
   int device_ = 0;

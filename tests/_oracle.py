@@ -18,7 +18,8 @@ class OraParams(C.Structure):
 
 class OraResult(C.Structure):
     _fields_ = [("status", C.c_int), ("solved", C.c_int), ("iterations", C.c_int), ("perturbed", C.c_int),
-                ("refactors", C.c_int), ("phase1_restarts", C.c_int), ("Z", C.c_double)]
+                ("refactors", C.c_int), ("phase1_restarts", C.c_int), ("Z", C.c_double),
+                ("phase1_iterations", C.c_int)]
 
 
 _lib = None
