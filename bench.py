@@ -28,6 +28,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel err≤1e-9"
 
+# HBM bytes per k_spmv launch on the default single-LP workload (m=20,000, n=40,000,
+# 10 nnz/col, seed 1, refactor every 64), from two rocprofv3 counter runs over iterations
+# 1..300: FETCH_SIZE 3,673.9 KiB x 1.9998 (a 256 MiB k_copy reads 131,087.1 KiB on that
+# counter) + WRITE_SIZE 316.7 KiB x 1.0 (the copy's writes read exactly). Counters cannot be
+# read inside the timed run, so other workloads report null.
+SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7847581}
+
 
 def log(msg):
     """Progress on stderr (the JSON result is the only stdout line)."""
@@ -249,6 +256,9 @@ def main():
     ap.add_argument("--warm-epochs", type=int, default=8)
     ap.add_argument("--warm-cpu-sample", type=int, default=64)
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a device buffer of this size once (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -303,7 +313,8 @@ def main():
                    "chunk": args.chunk, "status_at_end": r.status, "kernel_size_at_end": r.kernel_k},
         "roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_TRAFFIC_BYTES.get((args.m, args.n, args.k, args.seed, args.refactor)),
                      "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
                      "launches": r.spmv_launches},
     }
@@ -316,8 +327,22 @@ def main():
     if not args.no_warm:
         out["learned_warm_start"] = learned_warm(args.warm_train, args.warm_eval, args.warm_epochs, args.batch_reps,
                                                  0 if args.no_cpu_baseline else args.warm_cpu_sample)
+    if args.copy_calibration_mib > 0:
+        out["copy_calibration"] = copy_calibration(args.copy_calibration_mib)
     print(json.dumps(out), flush=True)
 
+
+def copy_calibration(mib):
+    """One plain copy kernel over `mib` MiB (k_copy): its FETCH_SIZE and WRITE_SIZE counters should
+    each read `mib` MiB, which gives the scale of those counters."""
+    import ctypes as C
+    from parallel_ds_amd._native import lib
+    ms = C.c_double()
+    rc = lib().ds_debug_copy(mib * 1024 * 1024, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError("ds_debug_copy failed: %d" % rc)
+    return {"bytes": mib * 1024 * 1024, "ms": round(ms.value, 4),
+            "gbs": round(2 * mib * 1024 * 1024 / (ms.value * 1e-3) / 1e9, 1)}
 
 if __name__ == "__main__":
     main()

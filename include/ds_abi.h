@@ -218,6 +218,10 @@ int ds_batch_get_warm(ds_ctx* ctx, int* basis /* sum m */);
 int ds_batch_set_repair_tol(ds_ctx* ctx, double tol);
 int ds_batch_repairs(ds_ctx* ctx, int* count /* per LP */);
 
+/* Counter calibration: one plain copy kernel over `bytes` (a multiple of 16)
+ * between two fresh device buffers; *ms = its duration. */
+int ds_debug_copy(long long bytes, double* ms);
+
 /* Trailing-update tile check: C(64x64, row-major) -= P(:, :nb) * B(nb x 64), where
  * P is 64x64 row-major (columns >= nb ignored) and B is nb x 64 row-major; the
  * FP64 MFMA kernel when use_mfma, else the plain FMA kernel. */

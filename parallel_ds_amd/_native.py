@@ -72,6 +72,7 @@ _SIGS = {
     "ds_pivot_row": (I32, [VP, VP, VP]),
     "ds_time_pivot_row": (I32, [VP, VP, I32, C.POINTER(F64), C.POINTER(F64)]),
     "ds_debug_gemm_tile": (I32, [VP, VP, VP, I32, I32]),
+    "ds_debug_copy": (I32, [C.c_longlong, C.POINTER(F64)]),
     "ds_batch_default_params": (None, [C.POINTER(BatchParams)]),
     "ds_batch_load": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP]),
     "ds_batch_set_warm": (I32, [VP, VP]),

@@ -5,6 +5,7 @@
 
 #include "ds_abi.h"
 #include "ds_batch_host.hpp"
+#include "ds_kernels.hpp"
 #include "ds_learn.hpp"
 #include "ds_solver.hpp"
 #include "ds_warm.hpp"
@@ -485,6 +486,33 @@ int ds_batch_set_repair_tol(ds_ctx* ctx, double tol) {
 int ds_batch_repairs(ds_ctx* ctx, int* count) {
   if (!count) return DS_EINVAL;
   return bguard(ctx, [&](pds::BatchSolver& B) { B.repairs(count); });
+}
+
+int ds_debug_copy(long long bytes, double* ms) {
+  if (bytes <= 0 || bytes % 16 != 0) return DS_EINVAL;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return DS_EHIP;
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return DS_EHIP;
+  }
+  int rc = DS_OK;
+  hipEvent_t e0, e1;
+  if (hipMemset(a, 0, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = DS_EHIP;
+  } else {
+    (void)hipEventRecord(e0, nullptr);
+    pds::launch_copy(nullptr, (const double2*)a, (double2*)b, bytes / 16);
+    (void)hipEventRecord(e1, nullptr);
+    float t = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = DS_EHIP;
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }
 
 int ds_debug_gemm_tile(double* C, const double* P, const double* B, int nb, int use_mfma) {

@@ -1012,6 +1012,12 @@ __global__ __launch_bounds__(BS) void k_fill(double* p, double v, int n) {
   if (i < n) p[i] = v;
 }
 
+// Plain streaming copy (16 B per lane per step): the reference kernel for
+// scaling the FETCH_SIZE / WRITE_SIZE hardware counters.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ a, double2* __restrict__ b, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) b[i] = a[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
@@ -1109,6 +1115,10 @@ void launch_dse_exact(hipStream_t s, const DevModel& M, const DevIter& I, const 
 }
 void launch_fill(hipStream_t s, double* p, double v, int n) {
   if (n > 0) k_fill<<<nblk(n, BS), BS, 0, s>>>(p, v, n);
+}
+
+void launch_copy(hipStream_t s, const double2* a, double2* b, long long n) {
+  k_copy<<<2048, 256, 0, s>>>(a, b, n);
 }
 
 }  // namespace pds

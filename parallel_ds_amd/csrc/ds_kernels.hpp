@@ -52,4 +52,7 @@ void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, con
 void launch_gj_panels(hipStream_t s, int k, const GJWork& G);
 void launch_gj_unscramble(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT);
 
+// Plain copy of n 16-byte elements (counter calibration).
+void launch_copy(hipStream_t s, const double2* a, double2* b, long long n);
+
 }  // namespace pds
