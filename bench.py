@@ -310,7 +310,8 @@ def main():
                                "from the slack basis" % (lp.m, lp.n, args.k, args.refactor, args.warmup + 1,
                                                          args.warmup + timed),
                    "m": lp.m, "n": lp.n, "nnz": lp.nnz, "seed": args.seed, "refactor_freq": args.refactor,
-                   "chunk": args.chunk, "status_at_end": r.status, "kernel_size_at_end": r.kernel_k},
+                   "chunk": args.chunk, "status_at_end": r.status, "kernel_size_at_end": r.kernel_k,
+                   "refactors": r.refactors, "refactor_ms_total": round(r.refactor_ms, 2)},
         "roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -35,7 +35,6 @@ void launch_fill(hipStream_t s, double* p, double v, int n);
 // Kernel inverse by blocked Gauss-Jordan (ds_refactor.hip).
 struct GJWork {
   double* W;      // k x k work matrix (row-major, ld = k)
-  double* Pf;     // k x NB panel LU factors
   double* Pb;     // k x NB saved panel
   int* piv;       // k
   int* perm;      // k

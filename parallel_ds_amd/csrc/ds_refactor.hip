@@ -30,31 +30,78 @@ __global__ __launch_bounds__(256) void k_build_kernel(DevModel M, const int* bas
   }
 }
 
-// Pivot rows of a panel by tournament: each workgroup factors up to TS_ROWS
-// rows of the panel W[., j:j+nb] (their current values) in LDS with partial
-// pivoting and keeps its nb pivot rows; rounds of merges (up to four lists
-// per workgroup) reduce the lists to one.  Level 0 reads rows j..k-1
-// directly (in_rows == nullptr).  The final workgroup also writes the LU of
-// the winners' block (Pf), its inverse (Pinv) and the row exchanges
-// piv[j..j+nb) that bring the winners to rows j..j+nb-1 in pivot order.
+// Pivot rows of a panel by tournament: each workgroup eliminates up to
+// TS_ROWS rows of the panel W[., j:j+nb] (their current values) with partial
+// pivoting and keeps its nb pivot rows in pivot order; rounds of merges (up to
+// four lists per workgroup) reduce the lists to one.  Level 0 reads rows
+// j..k-1 directly (in_rows == nullptr).  The final workgroup also writes the
+// inverse of the winners' block (Pinv) and the row exchanges piv[j..j+nb)
+// that bring the winners to rows j..j+nb-1 in pivot order.
+//
+// Layout: lane = row, a[x] = the row's panel column x (registers).  Steps run
+// in blocks of TS_BLK fully unrolled, and the register frame shifts by TS_BLK
+// columns after each block, so every register index is a constant.  A step:
+// wave max of |a[s]| over live rows, ballot for the lowest such row, the
+// wave's candidate lane writes its row to LDS (double-buffered by step
+// parity), one barrier, and every lane subtracts its multiple of the winner's
+// row.  Only the winners are used (the panel itself is eliminated by
+// k_mid_gemm / k_gemm_sub with Pinv), so the multipliers use the pivot's
+// reciprocal and pivoted rows are not kept.
 constexpr int TS_ROWS = 256;
-constexpr int TS_THREADS = 1024;
+constexpr int TS_THREADS = TS_ROWS;
+constexpr int TS_WAVES = TS_THREADS / WAVE;
+constexpr int TS_BLK = 16;
+constexpr int GJ_COLS = 2 * NB / TS_WAVES;  // [B | I] columns per wave in the final Gauss-Jordan
+static_assert(NB == WAVE, "one pivot-block row per lane");
+static_assert(NB % TS_BLK == 0 && NB % GJ_COLS == 0, "unrolled blocks tile the panel");
 
-// Pivot rows of a panel by tournament: each workgroup factors up to TS_ROWS
-// rows of the panel W[., j:j+nb] (their current values) in LDS with partial
-// pivoting and keeps its nb pivot rows; rounds of merges (up to four lists
-// per workgroup) reduce the lists to one.  Level 0 reads rows j..k-1
-// directly (in_rows == nullptr).  The final workgroup also writes the LU of
-// the winners' block (Pf), its inverse (Pinv) and the row exchanges
-// piv[j..j+nb) that bring the winners to rows j..j+nb-1 in pivot order.
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform_d(double v) {
+  const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+  const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+// DPP lane permutation of a double (both halves, every lane has a source).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// Wave maximum: quad_perm [1,0,3,2] and [2,3,0,1], row_ror 4 and 8 give each
+// row of 16 lanes its maximum; the four rows are combined from lanes 0/16/32/48.
+__device__ __forceinline__ double wave_max(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));
+  v = fmax(v, dpp_d<0x4E>(v));
+  v = fmax(v, dpp_d<0x124>(v));
+  v = fmax(v, dpp_d<0x128>(v));
+  return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+// 1/p from the hardware reciprocal and one Newton step (pivot choice only).
+__device__ __forceinline__ double recip_nr(double p) {
+  double r = __builtin_amdgcn_rcp(p);
+  return fma(r, fma(-p, r, 1.0), r);
+}
+
 __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int j, int nb, const int* in_rows,
                                                      const int* in_n, int in_groups, int* out_rows, int* out_n,
-                                                     int final_, double* Pf, double* Pinv, int* piv, int* flag) {
-  __shared__ double A[TS_ROWS][NB + 1];
+                                                     int final_, double* Pinv, int* piv, int* flag) {
   __shared__ int rid[TS_ROWS];
   __shared__ int goff[5];
+  __shared__ double cand[2][TS_WAVES][NB];
+  __shared__ double cv[2][TS_WAVES];
+  __shared__ int ci[2][TS_WAVES];
+  __shared__ int win[NB];
+  __shared__ int wrow[NB];
+  __shared__ double tile[TS_ROWS][NB + 1];  // the group's rows of the panel, as loaded
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
+  const int wv = uniform_i(tid / WAVE), lane = tid % WAVE;
   if (tid == 0) {
     int n = 0;
     goff[0] = 0;
@@ -78,128 +125,145 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
       rid[t] = in_rows[(4 * w + g) * NB + (t - goff[g])];
     }
   }
+  R = uniform_i(R);
   __syncthreads();
-  for (int e = tid; e < R * nb; e += TS_THREADS) {
-    int r = e / nb, c = e % nb;
-    A[r][c] = W[(size_t)rid[r] * k + j + c];
-  }
+  // Coalesced load (lane = column) into LDS, then lane = row into registers.
+#pragma unroll 8
+  for (int r = wv; r < R; r += TS_WAVES) tile[r][lane] = lane < nb ? W[(size_t)rid[r] * k + j + lane] : 0.0;
   __syncthreads();
+  bool live = tid < R;
+  double a[NB];
+#pragma unroll
+  for (int x = 0; x < NB; ++x) a[x] = live ? tile[tid][x] : 0.0;
   const int steps = min(nb, R);
-  for (int st = 0; st < steps; ++st) {
-    double bv = -1.0;
-    int bi = INT_MAX;
-    for (int r = st + tid; r < R; r += TS_THREADS) {
-      double a = fabs(A[r][st]);
-      if (a >= 0.0) argmax_merge(bv, bi, a, r);
-    }
-    block_argmax<TS_THREADS>(bv, bi);
-    if (bi == INT_MAX || !(bv > 0.0)) {
-      // No nonzero pivot in this column among these rows (common for a
-      // sparse kernel): leave the column, keep row st in place.  Only the
-      // final round makes this an error.
-      if (final_ && tid == 0) *flag = 1;
-      __syncthreads();
-      continue;
-    }
-    if (bi != st) {
-      for (int c = tid; c < nb; c += TS_THREADS) {
-        double x = A[st][c];
-        A[st][c] = A[bi][c];
-        A[bi][c] = x;
+  for (int cb = 0; cb * TS_BLK < steps; ++cb) {
+#pragma unroll
+    for (int s = 0; s < TS_BLK; ++s) {
+      const int st = cb * TS_BLK + s;
+      if (st < steps) {
+        const int b = st & 1;
+        // This wave's candidate: the live row with the largest |a[s]|, lowest
+        // row on ties (a NaN counts as 0, so a live row always qualifies).
+        double v = fabs(a[s]);
+        v = live ? (v == v ? v : 0.0) : -1.0;
+        const double m = uniform_d(wave_max(v));
+        const unsigned long long bal = __ballot(live && v == m);
+        const int cl = bal ? __ffsll((long long)bal) - 1 : -1;
+        if (lane == cl) {
+#pragma unroll
+          for (int x = s; x < NB; ++x) cand[b][wv][x] = a[x];
+        }
+        if (lane == 0) {
+          cv[b][wv] = m;
+          ci[b][wv] = cl >= 0 ? wv * WAVE + cl : INT_MAX;
+        }
+        __syncthreads();
+        double gv = -2.0;
+        int gr = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < TS_WAVES; ++q) {
+          const double v2 = cv[b][q];
+          const int r2 = ci[b][q];
+          const bool better = v2 > gv || (v2 == gv && r2 < gr);
+          gv = better ? v2 : gv;
+          gr = better ? r2 : gr;
+        }
+        gv = uniform_d(gv);
+        gr = uniform_i(gr);
+        // steps <= R, so some row is live and gr is a row of this group.
+        if (tid == 0) win[st] = gr;
+        if (tid == gr) live = false;
+        if (gv > 0.0) {
+          const double* pr = cand[b][gr / WAVE];
+          const double l = a[s] * recip_nr(pr[s]);
+#pragma unroll
+          for (int x = s + 1; x < NB; ++x) a[x] = a[x] - l * pr[x];
+        } else if (final_ && tid == 0) {
+          // No nonzero pivot in this column: only the final round makes it an error.
+          *flag = 1;
+        }
       }
-      if (tid == 0) {
-        int x = rid[st];
-        rid[st] = rid[bi];
-        rid[bi] = x;
-      }
     }
-    __syncthreads();
-    const double dv = A[st][st];
-    for (int r = st + 1 + tid; r < R; r += TS_THREADS) A[r][st] = A[r][st] / dv;
-    __syncthreads();
-    const int wdt = nb - st - 1;
-    if (wdt > 0)
-      for (int e = tid; e < (R - st - 1) * wdt; e += TS_THREADS) {
-        int r = st + 1 + e / wdt, c = st + 1 + e % wdt;
-        A[r][c] = A[r][c] - A[r][st] * A[st][c];
-      }
-    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < NB - TS_BLK; ++x) a[x] = a[x + TS_BLK];
+#pragma unroll
+    for (int x = NB - TS_BLK; x < NB; ++x) a[x] = 0.0;
   }
-  for (int t = tid; t < steps; t += TS_THREADS) out_rows[w * NB + t] = rid[t];
+  __syncthreads();
+  for (int t = tid; t < steps; t += TS_THREADS) {
+    wrow[t] = rid[win[t]];
+    out_rows[w * NB + t] = rid[win[t]];
+  }
   if (tid == 0) out_n[w] = steps;
   if (!final_) return;
   if (steps < nb && tid == 0) *flag = 1;
-  // Final round: LU factors, inverse of the winners' block, row exchanges.
-  for (int e = tid; e < nb * nb; e += TS_THREADS) Pf[(size_t)(e / nb) * NB + (e % nb)] = A[e / nb][e % nb];
   __syncthreads();
   // Gauss-Jordan on [B | I], B = the winners' original block in pivot order
-  // (same pivots as the LU above, so no further exchanges).
-  double(*G)[NB + 1] = &A[NB];       // rows NB..2NB-1: B
-  double(*H)[NB + 1] = &A[2 * NB];   // rows 2NB..3NB-1: I -> B^{-1}
-  __shared__ double prow[2 * NB];
-  __shared__ double fcol[NB];
-  for (int e = tid; e < nb * nb; e += TS_THREADS) {
-    int r = e / nb, c = e % nb;
-    G[r][c] = W[(size_t)rid[r] * k + j + c];
-    H[r][c] = (r == c) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int t = 0; t < nb; ++t) {
-    const double pv = G[t][t];
-    for (int c = tid; c < 2 * nb; c += TS_THREADS) prow[c] = (c < nb) ? G[t][c] : H[t][c - nb];
-    for (int r = tid; r < nb; r += TS_THREADS) fcol[r] = G[r][t] / pv;
-    __syncthreads();
-    for (int e = tid; e < nb * 2 * nb; e += TS_THREADS) {
-      int r = e / (2 * nb), c = e % (2 * nb);
-      double x = (c < nb) ? G[r][c] : H[r][c - nb];
-      x = (r == t) ? x / pv : x - fcol[r] * prow[c];
-      if (c < nb) G[r][c] = x;
-      else H[r][c - nb] = x;
+  // (the tournament's pivots, so no further exchanges).  Lane = row s; wave
+  // wv holds columns GJ_COLS*wv + x of [B | I].  Column t is in wave
+  // t / GJ_COLS, register t % GJ_COLS, a constant inside the unrolled block.
+  __shared__ double colt[2][NB];
+  __shared__ double rowt[2][2 * NB];
+  double g[GJ_COLS];
+  {
+    const int sl = lane < steps ? win[lane] : 0;
+#pragma unroll
+    for (int x = 0; x < GJ_COLS; ++x) {
+      const int c = GJ_COLS * wv + x;
+      double v = 0.0;
+      if (lane < steps) {
+        if (c < NB) v = tile[sl][c];
+        else v = (c - NB == lane) ? 1.0 : 0.0;
+      }
+      g[x] = v;
     }
-    __syncthreads();
   }
-  for (int e = tid; e < nb * nb; e += TS_THREADS) Pinv[(size_t)(e / nb) * NB + (e % nb)] = H[e / nb][e % nb];
-  // Sequential exchanges: position j+s receives winner rid[s].  Touched
-  // positions live in a small LDS map (position -> row now there), searched
-  // by wave 0.
-  __shared__ int key[2 * NB], val[2 * NB];
-  __shared__ int nk;
-  if (tid == 0) nk = 0;
-  __syncthreads();
-  if (tid < WAVE) {
-    const int lane = tid;
-    for (int s2 = 0; s2 < nb; ++s2) {
-      const int target = rid[s2];
-      const int pj = j + s2;
-      const int n = nk;
-      int fv = -1, fj = -1, fl = -1;
-      for (int q = lane; q < n; q += WAVE) {
-        if (val[q] == target) fv = q;
-        if (key[q] == pj) fj = q;
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        fv = max(fv, __shfl_xor(fv, o, WAVE));
-        fj = max(fj, __shfl_xor(fj, o, WAVE));
-      }
-      const int loc = (fv >= 0) ? key[fv] : target;
-      for (int q = lane; q < n; q += WAVE)
-        if (key[q] == loc) fl = q;
-      for (int o = 32; o > 0; o >>= 1) fl = max(fl, __shfl_xor(fl, o, WAVE));
-      if (lane == 0) {
-        const int cj = (fj >= 0) ? val[fj] : pj;
-        piv[pj] = loc;
-        int kj = fj, kl = fl, cnt = n;
-        if (kj < 0) { key[cnt] = pj; kj = cnt++; }
-        val[kj] = target;
-        if (loc != pj) {
-          if (kl < 0) { key[cnt] = loc; kl = cnt++; }
-          val[kl] = cj;
+  for (int tb = 0; tb * GJ_COLS < nb; ++tb) {
+#pragma unroll
+    for (int x = 0; x < GJ_COLS; ++x) {
+      const int t = tb * GJ_COLS + x;
+      if (t < nb) {
+        const int b = t & 1;
+        if (wv == tb) colt[b][lane] = g[x];
+        if (lane == t) {
+#pragma unroll
+          for (int y = 0; y < GJ_COLS; ++y) rowt[b][GJ_COLS * wv + y] = g[y];
         }
-        nk = cnt;
+        __syncthreads();
+        // Row t: f = 1 and pc == g[y], so the first FMA gives exactly 0 and
+        // the second pc / pv (as pc * rp); other rows add 0.
+        const double pv = rowt[b][t];
+        const double rp = 1.0 / pv;
+        const bool own = lane == t;
+        const double f = own ? 1.0 : colt[b][lane] * rp;
+        const double wt = own ? rp : 0.0;
+#pragma unroll
+        for (int y = 0; y < GJ_COLS; ++y) {
+          const double pc = rowt[b][GJ_COLS * wv + y];
+          g[y] = fma(wt, pc, fma(-f, pc, g[y]));
+        }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < GJ_COLS; ++x) {
+    const int c = GJ_COLS * wv + x - NB;
+    if (c >= 0 && c < nb && lane < nb) Pinv[(size_t)lane * NB + c] = g[x];
+  }
+  // Sequential exchanges: position j+s2 receives winner s2.  Lane i tracks
+  // winner i's current position (wh) and the row now at position j+i (oc).
+  if (wv == 0) {
+    const int myrow = lane < nb ? wrow[lane] : -1;
+    int wh = myrow;
+    int oc = j + lane;
+    for (int s2 = 0; s2 < nb; ++s2) {
+      const int loc = __builtin_amdgcn_readlane(wh, s2);
+      const int occ = __builtin_amdgcn_readlane(oc, s2);
+      if (lane == 0) piv[j + s2] = loc;
+      // the row at j+s2 moves to loc; if it is a later winner, so does its position
+      if (lane > s2 && lane < nb && myrow == occ) wh = loc;
+      if (lane == loc - j) oc = occ;
+      if (lane == s2) oc = myrow;
     }
   }
 }
@@ -374,14 +438,14 @@ void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
     int groups = (k - j + TS_ROWS - 1) / TS_ROWS;
     int* in_rows = G.rowsA;
     int* in_n = G.nA;
-    k_tslu<<<groups, TS_THREADS, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, groups == 1 ? 1 : 0, G.Pf,
-                                   G.Pinv, G.piv, G.flag);
+    k_tslu<<<groups, TS_THREADS, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, groups == 1 ? 1 : 0,
+                                         G.Pinv, G.piv, G.flag);
     int* out_rows = G.rowsB;
     int* out_n = G.nB;
     while (groups > 1) {
       int g2 = (groups + 3) / 4;
-      k_tslu<<<g2, TS_THREADS, 0, s>>>(G.W, k, j, nb, in_rows, in_n, groups, out_rows, out_n, g2 == 1 ? 1 : 0, G.Pf,
-                                 G.Pinv, G.piv, G.flag);
+      k_tslu<<<g2, TS_THREADS, 0, s>>>(G.W, k, j, nb, in_rows, in_n, groups, out_rows, out_n, g2 == 1 ? 1 : 0,
+                                       G.Pinv, G.piv, G.flag);
       int* tr = in_rows;
       in_rows = out_rows;
       out_rows = tr;

@@ -315,7 +315,6 @@ void Solver::load(const HostModel& hm) {
   piv_.alloc(m);
   perm_.alloc(m);
   flag_.alloc(1);
-  Pf_.alloc((size_t)m * GJ_NB);
   Pb_.alloc((size_t)m * GJ_NB);
   Pinv_.alloc((size_t)GJ_NB * GJ_NB);
   rowsA_.alloc((size_t)(m / 256 + 2) * GJ_NB);
@@ -412,7 +411,7 @@ void Solver::refactor() {
     int zero = 0;
     PDS_HIP(hipMemcpyAsync(flag_.p, &zero, sizeof(int), hipMemcpyHostToDevice, st_));
     launch_build_kernel(st_, M(), basis_.p, F(), GW_.p);
-    GJWork G{GW_.p, Pf_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_};
+    GJWork G{GW_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_};
     launch_gj_panels(st_, k, G);
     int flag = 0;
     std::vector<int> piv(k), perm(k);
