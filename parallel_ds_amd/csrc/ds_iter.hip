@@ -306,44 +306,65 @@ __global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFact
 
 // ---------------------------------------------------------------------------
 // Pivot-row SpMV: Matrix::dotUpdate UPDATE_T (matrix.cpp:273-287) over the
-// CSC of [A | I] (= CSR of A^T).  One wave per 64 consecutive columns: the
-// wave streams the columns' contiguous (row, value) segment, forms the
-// products into LDS, and each lane then sums its own column in CSC order, so
-// the result is bitwise that of the sequential loop.  Basic columns are
-// skipped at the store.  out index = nonbasic position, or the column id when
-// by_col (the `set` flag).
+// CSC of [A | I] (= CSR of A^T).  One 256-thread workgroup per SPMV_COLS
+// consecutive columns: the tile's column pointers and nonbasic positions go
+// to LDS, the workgroup streams the tile's contiguous (row, value) segment
+// with SPMV_ILP independent loads in flight per thread and writes the
+// products to LDS, and one lane per column then sums its column in CSC order,
+// so the result is bitwise that of the sequential loop.  Basic columns are
+// skipped at the store; so is everything when the solver has stopped (the
+// check sits after the loads, off their dependency chain).  out index =
+// nonbasic position, or the column id when by_col (the `set` flag).
 // ---------------------------------------------------------------------------
-constexpr int SPMV_SEG = 1024;
-__global__ __launch_bounds__(WAVE) void k_spmv(DevModel M, const int* __restrict__ posN, const double* __restrict__ v,
-                                               double alpha, double beta, const double* __restrict__ cvec,
-                                               double* __restrict__ out, int by_col, const DevScalars* S, int check) {
-  if (check && stopped(S)) return;
+constexpr int SPMV_THREADS = 256;
+constexpr int SPMV_COLS = 64;
+constexpr int SPMV_SEG = 2048;
+constexpr int SPMV_ILP = 4;
+static_assert(SPMV_COLS <= SPMV_THREADS, "one summing lane per column");
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __restrict__ posN,
+                                                       const double* __restrict__ v, double alpha, double beta,
+                                                       const double* __restrict__ cvec, double* __restrict__ out,
+                                                       int by_col, const DevScalars* S, int check) {
+  __shared__ int cp[SPMV_COLS + 1];
+  __shared__ int pn[SPMV_COLS];
   __shared__ double prod[SPMV_SEG];
-  const int lane = threadIdx.x;
-  const int j0 = blockIdx.x * WAVE;
-  const int jend = min(j0 + WAVE, M.N);
-  const int j = j0 + lane;
-  const int e0 = M.cptr[j0];
-  const int e1 = M.cptr[jend];
-  int pn = -1, cb = 0, ce = 0;
-  if (j < jend) {
-    pn = posN[j];
-    cb = M.cptr[j];
-    ce = M.cptr[j + 1];
-  }
-  if (e1 - e0 <= SPMV_SEG) {
-    for (int e = e0 + lane; e < e1; e += WAVE) prod[e - e0] = M.cval[e] * v[M.ridx[e]] * alpha;
-    __syncthreads();
-    if (j < jend && (pn >= 0)) {
-      double acc = 0.0;
-      for (int e = cb; e < ce; ++e) acc = acc + prod[e - e0];
-      out[by_col ? j : pn] = acc + cvec[j] * beta;
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * SPMV_COLS;
+  const int ncol = min(SPMV_COLS, M.N - j0);
+  if (tid <= ncol) cp[tid] = M.cptr[j0 + tid];
+  if (tid < ncol) pn[tid] = posN[j0 + tid];
+  __syncthreads();
+  const int e0 = cp[0], e1 = cp[ncol];
+  const bool staged = e1 - e0 <= SPMV_SEG;
+  if (staged) {
+    for (int base = e0; base < e1; base += SPMV_ILP * SPMV_THREADS) {
+      int ri[SPMV_ILP];
+      double cv[SPMV_ILP];
+#pragma unroll
+      for (int u = 0; u < SPMV_ILP; ++u) {
+        const int e = base + u * SPMV_THREADS + tid;
+        ri[u] = e < e1 ? M.ridx[e] : 0;
+        cv[u] = e < e1 ? M.cval[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < SPMV_ILP; ++u) {
+        const int e = base + u * SPMV_THREADS + tid;
+        if (e < e1) prod[e - e0] = cv[u] * v[ri[u]] * alpha;
+      }
     }
-  } else if (j < jend && pn >= 0) {
-    double acc = 0.0;
-    for (int e = cb; e < ce; ++e) acc = acc + M.cval[e] * v[M.ridx[e]] * alpha;
-    out[by_col ? j : pn] = acc + cvec[j] * beta;
   }
+  __syncthreads();
+  if (tid >= ncol || pn[tid] < 0) return;
+  if (check && stopped(S)) return;
+  const int cb = cp[tid], ce = cp[tid + 1];
+  double acc = 0.0;
+  if (staged) {
+    for (int e = cb; e < ce; ++e) acc = acc + prod[e - e0];
+  } else {
+    for (int e = cb; e < ce; ++e) acc = acc + M.cval[e] * v[M.ridx[e]] * alpha;
+  }
+  const int j = j0 + tid;
+  out[by_col ? j : pn[tid]] = acc + cvec[j] * beta;
 }
 
 // ---------------------------------------------------------------------------
@@ -1043,10 +1064,10 @@ void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double
                  const double* cvec, double* out, int by_col, const DevScalars* S, int check, hipEvent_t ev0,
                  hipEvent_t ev1) {
   if (ev0 && ev1)
-    hipExtLaunchKernelGGL(k_spmv, dim3(nblk(M.N, WAVE)), dim3(WAVE), 0, s, ev0, ev1, 0, M, posN, v, alpha, beta,
-                          cvec, out, by_col, S, check);
+    hipExtLaunchKernelGGL(k_spmv, dim3(nblk(M.N, SPMV_COLS)), dim3(SPMV_THREADS), 0, s, ev0, ev1, 0, M, posN, v,
+                          alpha, beta, cvec, out, by_col, S, check);
   else
-    k_spmv<<<nblk(M.N, WAVE), WAVE, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
+    k_spmv<<<nblk(M.N, SPMV_COLS), SPMV_THREADS, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
 }
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);
