@@ -144,6 +144,57 @@ def _oracle_family_solve(args):
     return _t.perf_counter() - t0, r.status, r.Z
 
 
+def transport(cpu, S=500, D=1000, seed=2, ratio_test=0, mark=200):
+    """Balanced transportation LP S x D (n = S*D, m = S+D, nnz = 2n) solved to optimality on the GPU.
+
+    SpMV roofline at this size (iterations after `mark`), objective against the HiGHS value of
+    the same instance, and the oracle's full solve of the same LP on one host core."""
+    from parallel_ds_amd import DualSimplex, default_params
+    from parallel_ds_amd import lp as lpmod
+    lp = lpmod.config2(seed=seed, S=S, D=D)
+    log("transport %dx%d: GPU solve" % (S, D))
+    prm = default_params(ratio_test=ratio_test, mark_iteration=mark, time_kernels=1)
+    with DualSimplex(lp, prm) as s:
+        s.initDualSimplex()
+        st = s.presolve("minInfeas")
+        if st == 0:
+            s.solve()
+        r = s.result()
+    spmv_avg_ms = r.spmv_ms / max(r.spmv_launches, 1)
+    nnzN = 0.5 * (r.nnz_nonbasic_mark + r.nnz_nonbasic_end)
+    spmv_bytes = 12.0 * nnzN + 4.0 * (lp.n + 1) + 8.0 * lp.n + 8.0 * lp.m
+    achieved = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
+    out = {"workload": "balanced transportation LP %d sources x %d sinks (m=%d, n=%d, nnz=%d), seed %d, "
+                       "solved from the slack basis" % (S, D, lp.m, lp.n, lp.nnz, seed),
+           "ratio_test": "harris" if ratio_test else "textbook",
+           "status": r.status, "objective": r.Z, "iterations": r.iterations, "perturbed": r.perturbed,
+           "solve_ms": round(r.solve_ms, 2), "ms_per_iter": round(r.solve_ms / max(r.iterations, 1), 4),
+           "solves_per_s": round(1e3 / r.solve_ms, 3) if r.solve_ms > 0 else None,
+           "spmv_roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm", "achieved": round(achieved, 2),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
+                             "launches": r.spmv_launches}}
+    gold = os.path.join(ROOT, "tests", "golden", "highs_objectives.json")
+    key = "transport_%dx%d_seed%d" % (S, D, seed)
+    if os.path.exists(gold):
+        g = json.load(open(gold)).get(key)
+        if g:
+            out["highs_objective"] = g["objective"]
+            out["rel_err_vs_highs"] = abs(r.Z - g["objective"]) / max(abs(g["objective"]), 1.0)
+    if cpu:
+        log("transport %dx%d: CPU oracle full solve" % (S, D))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as ora  # test-side binding of the oracle
+        t0 = time.perf_counter()
+        ro, _, _ = ora.solve(lp, ratio_test=ratio_test)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / sec, 4), "unit": "solves/s", "solve_s": round(sec, 2),
+                               "ms_per_iter": round(1e3 * sec / max(ro.iterations, 1), 4), "status": ro.status,
+                               "objective": ro.Z, "iterations": ro.iterations, "cores": 1, "kind": "port",
+                               "sample": "oracle full solve of the same LP, 1 thread, %s" % cpu_model()}
+    return out
+
+
 def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, seed=4, sigma=0.1):
     """Learned warm start vs slack start on the same GPU: an LP family of the single-LP recipe
     (k nonzeros per column) whose b and c are perturbed by lognormal(0, sigma); the network is
@@ -256,6 +307,7 @@ def main():
     ap.add_argument("--warm-epochs", type=int, default=8)
     ap.add_argument("--warm-cpu-sample", type=int, default=64)
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-transport", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size once (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -325,6 +377,8 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_family(args.batch_count, 256, args.batch_reps,
                                         0 if args.no_cpu_baseline else args.batch_cpu_sample)
+    if not args.no_transport:
+        out["transport"] = transport(not args.no_cpu_baseline)
     if not args.no_warm:
         out["learned_warm_start"] = learned_warm(args.warm_train, args.warm_eval, args.warm_epochs, args.batch_reps,
                                                  0 if args.no_cpu_baseline else args.warm_cpu_sample)

@@ -317,22 +317,29 @@ __global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFact
 // nonbasic position, or the column id when by_col (the `set` flag).
 // ---------------------------------------------------------------------------
 constexpr int SPMV_THREADS = 256;
-constexpr int SPMV_COLS = 64;
+constexpr int SPMV_COLS_MAX = 512;
 constexpr int SPMV_SEG = 2048;
 constexpr int SPMV_ILP = 4;
-static_assert(SPMV_COLS <= SPMV_THREADS, "one summing lane per column");
+constexpr int SPMV_GRID = 1024;  // tile width: the fewest columns that keep the grid within this
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __restrict__ posN,
                                                        const double* __restrict__ v, double alpha, double beta,
                                                        const double* __restrict__ cvec, double* __restrict__ out,
-                                                       int by_col, const DevScalars* S, int check) {
-  __shared__ int cp[SPMV_COLS + 1];
-  __shared__ int pn[SPMV_COLS];
+                                                       int by_col, const DevScalars* S, int check, int cols) {
+  __shared__ int cp[SPMV_COLS_MAX + 1];
+  __shared__ int pn[SPMV_COLS_MAX];
+  __shared__ double cb[SPMV_COLS_MAX];
   __shared__ double prod[SPMV_SEG];
+  __shared__ int halt;
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * SPMV_COLS;
-  const int ncol = min(SPMV_COLS, M.N - j0);
-  if (tid <= ncol) cp[tid] = M.cptr[j0 + tid];
-  if (tid < ncol) pn[tid] = posN[j0 + tid];
+  const int j0 = blockIdx.x * cols;
+  const int ncol = min(cols, M.N - j0);
+  // Everything that does not depend on the entries is fetched in one round.
+  for (int c = tid; c <= ncol; c += SPMV_THREADS) cp[c] = M.cptr[j0 + c];
+  for (int c = tid; c < ncol; c += SPMV_THREADS) {
+    pn[c] = posN[j0 + c];
+    cb[c] = cvec[j0 + c];
+  }
+  if (tid == 0) halt = check && stopped(S);
   __syncthreads();
   const int e0 = cp[0], e1 = cp[ncol];
   const bool staged = e1 - e0 <= SPMV_SEG;
@@ -354,17 +361,26 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __
     }
   }
   __syncthreads();
-  if (tid >= ncol || pn[tid] < 0) return;
-  if (check && stopped(S)) return;
-  const int cb = cp[tid], ce = cp[tid + 1];
-  double acc = 0.0;
-  if (staged) {
-    for (int e = cb; e < ce; ++e) acc = acc + prod[e - e0];
-  } else {
-    for (int e = cb; e < ce; ++e) acc = acc + M.cval[e] * v[M.ridx[e]] * alpha;
+  if (halt) return;
+  for (int c = tid; c < ncol; c += SPMV_THREADS) {
+    if (pn[c] < 0) continue;
+    const int eb = cp[c], ee = cp[c + 1];
+    double acc = 0.0;
+    if (staged) {
+      for (int e = eb; e < ee; ++e) acc = acc + prod[e - e0];
+    } else {
+      for (int e = eb; e < ee; ++e) acc = acc + M.cval[e] * v[M.ridx[e]] * alpha;
+    }
+    out[by_col ? j0 + c : pn[c]] = acc + cb[c] * beta;
   }
-  const int j = j0 + tid;
-  out[by_col ? j : pn[tid]] = acc + cvec[j] * beta;
+}
+
+// Tile width: the smallest power of two in [64, SPMV_COLS_MAX] whose grid has
+// at most SPMV_GRID workgroups (about four per CU).
+static int spmv_cols(const DevModel& M) {
+  int c = 64;
+  while (c < SPMV_COLS_MAX && (M.N + c - 1) / c > SPMV_GRID) c *= 2;
+  return c;
 }
 
 // ---------------------------------------------------------------------------
@@ -1063,11 +1079,12 @@ void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
                  const double* cvec, double* out, int by_col, const DevScalars* S, int check, hipEvent_t ev0,
                  hipEvent_t ev1) {
+  const int cols = spmv_cols(M);
   if (ev0 && ev1)
-    hipExtLaunchKernelGGL(k_spmv, dim3(nblk(M.N, SPMV_COLS)), dim3(SPMV_THREADS), 0, s, ev0, ev1, 0, M, posN, v,
-                          alpha, beta, cvec, out, by_col, S, check);
+    hipExtLaunchKernelGGL(k_spmv, dim3(nblk(M.N, cols)), dim3(SPMV_THREADS), 0, s, ev0, ev1, 0, M, posN, v, alpha,
+                          beta, cvec, out, by_col, S, check, cols);
   else
-    k_spmv<<<nblk(M.N, SPMV_COLS), SPMV_THREADS, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check);
+    k_spmv<<<nblk(M.N, cols), SPMV_THREADS, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check, cols);
 }
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);

@@ -149,3 +149,23 @@ def test_btran_ftran_large_kernel(gpu, seed):
             e[p] = 1
             ref = np.linalg.solve(B.T, e)
             assert np.abs(rho - ref).max() <= 1e-9 * max(1, np.abs(ref).max())
+
+
+def test_transport_full_size_vs_highs(gpu):
+    """The balanced transportation LP at full size (500 x 1000: m = 1,500, n = 500,000, nnz = 10^6),
+    solved from the slack basis: optimal, objective equal to HiGHS's to 1e-9 relative. The oracle
+    is too slow for a test at this size (about 10 s); the bench times it."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "highs_objectives.json")))
+    ref = gold["transport_500x1000_seed2"]["objective"]
+    L = lpmod.config2(seed=2, S=500, D=1000)
+    assert (L.m, L.n, L.nnz) == (1500, 500000, 1000000)
+    with DualSimplex(L, CAP) as s:
+        s.initDualSimplex()
+        assert s.presolve("minInfeas") == 0
+        s.solve()
+        res = s.result()
+        sol = s.getSolution()
+    assert res.status == DS_OPTIMAL and sol.solved
+    assert abs(sol.Z - ref) <= 1e-9 * abs(ref)
