@@ -144,6 +144,31 @@ def _oracle_family_solve(args):
     return _t.perf_counter() - t0, r.status, r.Z
 
 
+def spmv_sweep(reps=100, flush_mib=640):
+    """Pivot-row SpMV alone at three sizes (C4 base -> C1 -> C2), slack basis (every structural
+    column nonbasic), warm and cold (caches flushed by a copy of flush_mib MiB before each launch)."""
+    import numpy as np
+    from parallel_ds_amd import DualSimplex, default_params
+    from parallel_ds_amd import lp as lpmod
+    cases = [("family base m=2000 n=5000 5 nnz/col", lpmod.config1(seed=4, m=2000, n=5000, k=5)),
+             ("single LP m=20000 n=40000 10 nnz/col", lpmod.config1(seed=1)),
+             ("transportation LP 500x1000 (m=1500, n=500000)", lpmod.config2(seed=2))]
+    rows = []
+    for name, lp in cases:
+        log("SpMV sweep: %s" % name)
+        rho = np.random.default_rng(0).standard_normal(lp.m)
+        with DualSimplex(lp, default_params()) as s:
+            s.initDualSimplex()
+            w, c, by = s.time_pivot_row_ext(rho, reps, flush_mib << 20)
+        rows.append({"workload": name, "bytes_per_launch": round(by),
+                     "warm_us": round(w * 1e3, 3), "warm_gbs": round(by / (w * 1e-3) / 1e9, 1),
+                     "warm_frac": round(by / (w * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "cold_us": round(c * 1e3, 3), "cold_gbs": round(by / (c * 1e-3) / 1e9, 1),
+                     "cold_frac": round(by / (c * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+    return {"kernel": "k_spmv (pivot-row SpMV)", "reps": reps, "flush_mib": flush_mib, "peak_gbs": HBM_PEAK_GBS,
+            "sizes": rows}
+
+
 def transport(cpu, S=500, D=1000, seed=2, ratio_test=0, mark=200):
     """Balanced transportation LP S x D (n = S*D, m = S+D, nnz = 2n) solved to optimality on the GPU.
 
@@ -308,6 +333,7 @@ def main():
     ap.add_argument("--warm-cpu-sample", type=int, default=64)
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-transport", action="store_true")
+    ap.add_argument("--no-spmv-sweep", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size once (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -377,6 +403,8 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_family(args.batch_count, 256, args.batch_reps,
                                         0 if args.no_cpu_baseline else args.batch_cpu_sample)
+    if not args.no_spmv_sweep:
+        out["spmv_sweep"] = spmv_sweep()
     if not args.no_transport:
         out["transport"] = transport(not args.no_cpu_baseline)
     if not args.no_warm:

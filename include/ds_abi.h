@@ -129,6 +129,14 @@ int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis /* n */);
  * launch in *ms and the algorithmic bytes of one launch in *bytes. */
 int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, double* bytes);
 
+/* Per-launch timing of the pivot-row SpMV (start/stop events around each of
+ * `reps` launches): *warm_ms back to back, as inside the solver; *cold_ms with
+ * `flush_bytes` streamed through a copy kernel before each launch (0 skips the
+ * cold pass and returns 0).  *bytes = algorithmic bytes of one launch.
+ * Measurement entry; no counterpart in the reference. */
+int ds_time_pivot_row_ext(ds_ctx* ctx, const double* rho, int reps, long long flush_bytes, double* warm_ms,
+                          double* cold_ms, double* bytes);
+
 /* ---- Batched small LPs: one persistent launch, one workgroup per LP ----
  * No counterpart in the reference (it solves one LP per process,
  * src/main.cpp:55-92); each LP follows the same algorithm and rules as

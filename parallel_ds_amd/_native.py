@@ -71,6 +71,7 @@ _SIGS = {
     "ds_ftran_column": (I32, [VP, I32, VP]),
     "ds_pivot_row": (I32, [VP, VP, VP]),
     "ds_time_pivot_row": (I32, [VP, VP, I32, C.POINTER(F64), C.POINTER(F64)]),
+    "ds_time_pivot_row_ext": (I32, [VP, VP, I32, C.c_longlong, C.POINTER(F64), C.POINTER(F64), C.POINTER(F64)]),
     "ds_debug_gemm_tile": (I32, [VP, VP, VP, I32, I32]),
     "ds_debug_copy": (I32, [C.c_longlong, C.POINTER(F64)]),
     "ds_batch_default_params": (None, [C.POINTER(BatchParams)]),

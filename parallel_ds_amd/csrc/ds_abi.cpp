@@ -264,6 +264,12 @@ int ds_time_pivot_row(ds_ctx* ctx, const double* rho, int reps, double* ms, doub
   return guard(ctx, [&](pds::Solver& s) { s.time_pivot_row(rho, reps, *ms, *bytes); });
 }
 
+int ds_time_pivot_row_ext(ds_ctx* ctx, const double* rho, int reps, long long flush_bytes, double* warm_ms,
+                          double* cold_ms, double* bytes) {
+  if (!rho || !warm_ms || !cold_ms || !bytes || reps <= 0 || flush_bytes < 0) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.time_pivot_row_ext(rho, reps, flush_bytes, *warm_ms, *cold_ms, *bytes); });
+}
+
 void ds_batch_default_params(ds_batch_params* p) {
   if (!p) return;
   p->refactor_freq = 150;

@@ -88,17 +88,23 @@ __device__ inline double block_min(double v) {
 
 // Last-workgroup detection: every workgroup of the launch must call it once.
 // Returns true in the workgroup that arrives last; that workgroup then sees
-// all partials the others wrote before arriving.
+// all partials the others wrote before arriving.  The partials must be
+// written by wave 0 (thread 0's wave), which issues the release: an
+// agent-scope release fence (L2 writeback) before a relaxed count, and the
+// last workgroup's agent-scope acquire fence after it.  The producers do not
+// invalidate their L2 (a sequentially consistent fence would, in every
+// workgroup, and the following loads of the launch and of the next kernels
+// would miss).
 __device__ inline bool last_block(int* ctr) {
   __shared__ int s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    int prev = atomicAdd(ctr, 1);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const int prev = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == (int)gridDim.x - 1);
   }
   __syncthreads();
-  if (s_last) __threadfence();
+  if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_last != 0;
 }
 

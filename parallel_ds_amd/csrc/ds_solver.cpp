@@ -202,7 +202,7 @@ DevIter Solver::I() const {
   it.S = S_.p;
   it.part_d = part_d_.p;
   it.part_i = part_i_.p;
-  it.part_cap = 2048;
+  it.part_cap = 512;  // workgroups of a reduction launch: each one pays an L2 writeback at its release
   return it;
 }
 
@@ -976,6 +976,48 @@ void Solver::pivot_row(const double* rho, double* alpha) {
   launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0);
   PDS_HIP(hipMemcpyAsync(alpha, alpha_r_.p, sizeof(double) * H_.n, hipMemcpyDeviceToHost, st_));
   sync();
+}
+
+void Solver::time_pivot_row_ext(const double* rho, int reps, long long flush_bytes, double& warm_ms, double& cold_ms,
+                                double& bytes) {
+  // Per-launch kernel time from start/stop events around each SpMV launch:
+  // warm = back to back (the matrix stays in L2/MALL, as inside the solver);
+  // cold = after streaming flush_bytes through a copy kernel before each one.
+  PDS_HIP(hipMemcpyAsync(rho_.p, rho, sizeof(double) * H_.m, hipMemcpyHostToDevice, st_));
+  launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0);
+  hipEvent_t e0, e1;
+  PDS_HIP(hipEventCreate(&e0));
+  PDS_HIP(hipEventCreate(&e1));
+  double tw = 0.0, tc = 0.0;
+  float t = 0.f;
+  for (int r = 0; r < reps; ++r) {
+    launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0, e0, e1);
+    PDS_HIP(hipEventSynchronize(e1));
+    PDS_HIP(hipEventElapsedTime(&t, e0, e1));
+    tw += t;
+  }
+  if (flush_bytes >= 16) {
+    const long long n16 = flush_bytes / 16;
+    DBuf<double> fa, fb;
+    fa.alloc((size_t)n16 * 2);
+    fb.alloc((size_t)n16 * 2);
+    PDS_HIP(hipMemsetAsync(fa.p, 0, (size_t)n16 * 16, st_));
+    for (int r = 0; r < reps; ++r) {
+      launch_copy(st_, (const double2*)fa.p, (double2*)fb.p, n16);
+      launch_spmv(st_, M(), posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 0, e0, e1);
+      PDS_HIP(hipEventSynchronize(e1));
+      PDS_HIP(hipEventElapsedTime(&t, e0, e1));
+      tc += t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  warm_ms = reps > 0 ? tw / reps : 0.0;
+  cold_ms = (reps > 0 && flush_bytes >= 16) ? tc / reps : 0.0;
+  std::vector<int> nb = nonbasis_host();
+  double nnzN = 0;
+  for (int j : nb) nnzN += H_.cptr[j + 1] - H_.cptr[j];
+  bytes = 12.0 * nnzN + 4.0 * (nb.size() + 1) + 8.0 * nb.size() + 8.0 * H_.m;
 }
 
 void Solver::time_pivot_row(const double* rho, int reps, double& ms, double& bytes) {

@@ -88,6 +88,8 @@ class Solver {
   void ftran_column(int j, double* x);
   void pivot_row(const double* rho, double* alpha);
   void time_pivot_row(const double* rho, int reps, double& ms, double& bytes);
+  void time_pivot_row_ext(const double* rho, int reps, long long flush_bytes, double& warm_ms, double& cold_ms,
+                          double& bytes);
 
   std::string last_error;
 

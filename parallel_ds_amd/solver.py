@@ -144,6 +144,14 @@ class DualSimplex:
                     "ds_time_pivot_row")
         return ms.value, by.value
 
+    def time_pivot_row_ext(self, rho, reps=100, flush_bytes=640 << 20):
+        """(warm ms, cold ms, algorithmic bytes) per SpMV launch; cold flushes caches before each launch."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        w, c, by = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().ds_time_pivot_row_ext(self._ctx, ptr(rho), reps, flush_bytes, C.byref(w), C.byref(c),
+                                                C.byref(by)), "ds_time_pivot_row_ext")
+        return w.value, c.value, by.value
+
 
 def solve_lp(lp, params=None, device=0):
     """load + initDualSimplex + presolve("minInfeas") + solve; returns (LPsolution, basis, Result)."""

@@ -169,3 +169,14 @@ def test_transport_full_size_vs_highs(gpu):
         sol = s.getSolution()
     assert res.status == DS_OPTIMAL and sol.solved
     assert abs(sol.Z - ref) <= 1e-9 * abs(ref)
+
+
+def test_time_pivot_row_ext(gpu):
+    """The SpMV timing entry: positive warm and cold per-launch times, algorithmic bytes of the slack basis."""
+    L = lpmod.config0(2)
+    with DualSimplex(L, CAP) as s:
+        s.initDualSimplex()
+        w, c, by = s.time_pivot_row_ext(np.ones(L.m), reps=5, flush_bytes=16 << 20)
+        w2, c2, _ = s.time_pivot_row_ext(np.ones(L.m), reps=3, flush_bytes=0)
+    assert w > 0 and c > 0 and w2 > 0 and c2 == 0
+    assert by == 12.0 * L.nnz + 4.0 * (L.n + 1) + 8.0 * L.n + 8.0 * L.m
