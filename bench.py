@@ -28,12 +28,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel err≤1e-9"
 
-# HBM bytes per k_spmv launch on the default single-LP workload (m=20,000, n=40,000,
-# 10 nnz/col, seed 1, refactor every 64), from two rocprofv3 counter runs over iterations
-# 1..300: FETCH_SIZE 3,673.9 KiB x 1.9998 (a 256 MiB k_copy reads 131,087.1 KiB on that
-# counter) + WRITE_SIZE 316.7 KiB x 1.0 (the copy's writes read exactly). Counters cannot be
-# read inside the timed run, so other workloads report null.
-SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7847581}
+# HBM bytes per k_spmv launch, from two rocprofv3 counter runs (FETCH_SIZE, then WRITE_SIZE), each
+# scaled by what it reads for k_copy moving 256 MiB (FETCH_SIZE reads 131,087.8 KiB of the
+# 262,144 KiB read: x 1.99976; WRITE_SIZE reads the copy's writes exactly: x 1.0).  Counters
+# cannot be read inside a timed run, so other workloads report null.
+#   single LP m=20,000 n=40,000, 10 nnz/col, seed 1, refactor every 64 (iterations 1..300):
+#     3,635.3 KiB x 1.99976 + 316.7 KiB
+#   transportation LP 500 x 1000, seed 2 (all 2,294 iterations): 9,916.4 KiB x 1.99976 + 3,849.1 KiB
+SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7768498}
+TRANSPORT_SPMV_TRAFFIC_BYTES = {(500, 1000, 2): 24247818}
 
 
 def log(msg):
@@ -197,6 +200,7 @@ def transport(cpu, S=500, D=1000, seed=2, ratio_test=0, mark=200):
            "solves_per_s": round(1e3 / r.solve_ms, 3) if r.solve_ms > 0 else None,
            "spmv_roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm", "achieved": round(achieved, 2),
                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             "traffic": TRANSPORT_SPMV_TRAFFIC_BYTES.get((S, D, seed)),
                              "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
                              "launches": r.spmv_launches}}
     gold = os.path.join(ROOT, "tests", "golden", "highs_objectives.json")
