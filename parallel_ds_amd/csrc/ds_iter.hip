@@ -148,6 +148,7 @@ __global__ __launch_bounds__(BS) void k_dualcheck(DevModel M, DevIter I, int it)
 // one parallel matvec (b_k = eta_k[p] while position p still holds its unit
 // entry).  The kernel-inverse gather then runs on all rows in k_btran_out.
 // ---------------------------------------------------------------------------
+constexpr int MF_TILE = 64;  // eta counts up to this keep MF's lower triangle in LDS
 __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed) {
   DevScalars* S = I.S;
   if (p_fixed < 0 && stopped(S)) return;
@@ -161,37 +162,70 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   __shared__ double u[ETA_MAX];
   __shared__ int cnt[ETA_MAX + 1];
   __shared__ int off[ETA_MAX + 2];
+  __shared__ int pidx[ETA_MAX + 1];  // kernel column of the position, -1 at a slack position
+  __shared__ int srow[ETA_MAX + 1];  // slack row of a slack position
+  __shared__ int ccb[ETA_MAX + 1], cce[ETA_MAX + 1];  // its coupling-row extent
+  __shared__ double mf[MF_TILE * MF_TILE];
   __shared__ int kmax_p, kmin_p;
   const int ns = t + 1;
-  if (threadIdx.x == 0) {
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+  const bool tile = t <= MF_TILE;
+  if (tid == 0) {
     kmax_p = -1;
     kmin_p = INT_MAX;
   }
-  for (int s = threadIdx.x; s < ns; s += BS) pos[s] = (s == 0) ? p : I.eta_pos[s - 1];
+  for (int s = tid; s < ns; s += BS) pos[s] = (s == 0) ? p : I.eta_pos[s - 1];
+  // MF's lower triangle (the part the sweep reads) into LDS when it fits.
+  if (tile)
+    for (int e = tid; e < t * MF_TILE; e += BS) {
+      const int r = e / MF_TILE, c = e % MF_TILE;
+      mf[e] = (c <= r) ? I.MF[(size_t)r * cap + c] : 0.0;
+    }
   __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS)
+  for (int k = tid; k < t; k += BS)
     if (pos[1 + k] == p) {
       atomicMax(&kmax_p, k);
       atomicMin(&kmin_p, k);
     }
-  for (int s = threadIdx.x; s < ns; s += BS) {
+  for (int s = tid; s < ns; s += BS) {
+    const int ps = pos[s];
+    const int c = F.Pidx_of_pos[ps];
+    pidx[s] = c;
+    if (c < 0) {
+      const int r = F.slackrow_of_pos[ps];
+      srow[s] = r;
+      ccb[s] = F.crp[r];
+      cce[s] = F.crp[r + 1];
+    }
+  }
+  // slot[s] = the first s2 <= s at the same position (one s per wave at a time).
+  for (int s = wv; s < ns; s += BS / WAVE) {
     int f = s;
-    for (int s2 = 0; s2 < s; ++s2)
-      if (pos[s2] == pos[s]) { f = s2; break; }
-    slot[s] = f;
+    for (int b0 = 0; b0 < s; b0 += WAVE) {
+      const int s2 = b0 + lane;
+      const unsigned long long bal = __ballot(s2 < s && pos[s2] == pos[s]);
+      if (bal) {
+        f = b0 + __ffsll((long long)bal) - 1;
+        break;
+      }
+    }
+    if (lane == 0) slot[s] = f;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS) bv[k] = (k >= kmax_p) ? I.etas[(size_t)k * m + p] : 0.0;
+  for (int k = tid; k < t; k += BS) bv[k] = (k >= kmax_p) ? I.etas[(size_t)k * m + p] : 0.0;
   __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS) {
+  for (int k = tid; k < t; k += BS) {
     double acc = 0.0;
-    for (int k2 = k; k2 < t; ++k2) acc += I.MF[(size_t)k2 * cap + k] * bv[k2];
+    if (tile)
+      for (int k2 = k; k2 < t; ++k2) acc += mf[k2 * MF_TILE + k] * bv[k2];
+    else
+      for (int k2 = k; k2 < t; ++k2) acc += I.MF[(size_t)k2 * cap + k] * bv[k2];
     u[k] = acc;
   }
   __syncthreads();
   // Final values at the support: each position keeps the last assignment of
   // the reverse sweep, i.e. that of its smallest eta index.
-  for (int s = threadIdx.x; s < ns; s += BS) {
+  for (int s = tid; s < ns; s += BS) {
     double val = 0.0;
     if (slot[s] == s) {
       if (s == 0) val = (kmax_p < 0) ? 1.0 : u[kmin_p];
@@ -202,64 +236,69 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   __syncthreads();
   // Right-hand side of the kernel solve: rhs_P = v_P - C^T y_S, with
   // y_S = v at slack positions (ordered by slot, prefix-summed offsets).
-  for (int s = threadIdx.x; s < ns; s += BS) {
+  for (int s = tid; s < ns; s += BS) {
     int c = 0;
-    if (slot[s] == s && v[s] != 0.0) {
-      int ps = pos[s];
-      if (F.Pidx_of_pos[ps] >= 0) c = 1;
-      else {
-        int r = F.slackrow_of_pos[ps];
-        c = 1 + (F.crp[r + 1] - F.crp[r]);
-      }
-    }
+    if (slot[s] == s && v[s] != 0.0) c = pidx[s] >= 0 ? 1 : 1 + (cce[s] - ccb[s]);
     cnt[s] = c;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int a = 0;
-    for (int s = 0; s < ns; ++s) { off[s] = a; a += cnt[s]; }
-    off[ns] = a;
+  if (wv == 0) {  // exclusive prefix sum of cnt
+    int carry = 0;
+    for (int b0 = 0; b0 < ns; b0 += WAVE) {
+      const int s = b0 + lane;
+      const int c = s < ns ? cnt[s] : 0;
+      int x = c;
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+      }
+      if (s < ns) off[s] = carry + x - c;
+      carry += __shfl(x, WAVE - 1, WAVE);
+    }
+    if (lane == 0) off[ns] = carry;
   }
   __syncthreads();
   const int total = off[ns];
   if (total > I.list_cap) {
-    if (threadIdx.x == 0) raise_status(S, ST_BTRAN_CAP, -1);
+    if (tid == 0) raise_status(S, ST_BTRAN_CAP, -1);
     return;
   }
-  for (int s = threadIdx.x; s < ns; s += BS) {
+  for (int s = tid; s < ns; s += BS) {
     if (cnt[s] == 0) continue;
-    int ps = pos[s];
-    double vs = v[s];
+    const double vs = v[s];
     int o = off[s];
-    int c = F.Pidx_of_pos[ps];
+    const int c = pidx[s];
     if (c >= 0) {
       I.list_idx[o] = c;
       I.list_val[o] = vs;
     } else {
-      int r = F.slackrow_of_pos[ps];
+      const int r = srow[s];
       I.list_idx[o] = -1 - r;   // marks a slack row entry (no kernel contribution)
       I.list_val[o] = vs;
-      for (int e = F.crp[r]; e < F.crp[r + 1]; ++e) {
+      for (int e = ccb[s]; e < cce[s]; ++e) {
         ++o;
         I.list_idx[o] = F.cci[e];
         I.list_val[o] = -F.ccv[e] * vs;
       }
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  if (wv == 0) {  // slack positions in order, compacted by ballot
     int a = 0;
-    for (int s = 0; s < ns; ++s) {
-      if (cnt[s] == 0) continue;
-      int c = F.Pidx_of_pos[pos[s]];
-      if (c < 0) {
-        I.ylist_row[a] = F.slackrow_of_pos[pos[s]];
-        I.ylist_val[a] = v[s];
-        ++a;
+    for (int b0 = 0; b0 < ns; b0 += WAVE) {
+      const int s = b0 + lane;
+      const bool take = s < ns && cnt[s] != 0 && pidx[s] < 0;
+      const unsigned long long bal = __ballot(take);
+      if (take) {
+        const int idx = a + __popcll(bal & ((1ull << lane) - 1ull));
+        I.ylist_row[idx] = srow[s];
+        I.ylist_val[idx] = v[s];
       }
+      a += __popcll(bal);
     }
-    I.list_n[0] = total;
-    I.list_n[1] = a;
+    if (lane == 0) {
+      I.list_n[0] = total;
+      I.list_n[1] = a;
+    }
   }
 }
 
@@ -618,26 +657,28 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   __shared__ int er[FT_MAXCOL];
   __shared__ int eri[FT_MAXCOL];
   __shared__ double ev[FT_MAXCOL];
+  // The row's own factor data does not depend on the column: fetch it first.
+  const int i = blockIdx.x * BS + threadIdx.x;
+  const bool live = i < M.m;
+  const int c = live ? F.Pidx_of_pos[i] : 0;
+  const int r = (live && c < 0) ? F.slackrow_of_pos[i] : 0;
   const int cb = M.cptr[j];
   const int ne = min(M.cptr[j + 1] - cb, FT_MAXCOL);
   for (int e = threadIdx.x; e < ne; e += BS) {
-    int r = M.ridx[cb + e];
-    er[e] = r;
-    eri[e] = F.Ridx_of_row[r];
+    int rr = M.ridx[cb + e];
+    er[e] = rr;
+    eri[e] = F.Ridx_of_row[rr];
     ev[e] = M.cval[cb + e];
   }
   __syncthreads();
-  const int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= M.m) return;
+  if (!live) return;
   const int k = F.k;
-  int c = F.Pidx_of_pos[i];
   double x;
   if (c >= 0) {
     x = 0.0;
     for (int e = 0; e < ne; ++e)
       if (eri[e] >= 0) x += ev[e] * F.KinvT[(size_t)eri[e] * k + c];
   } else {
-    int r = F.slackrow_of_pos[i];
     double ar = 0.0;
     for (int e = 0; e < ne; ++e)
       if (er[e] == r) ar = ev[e];

@@ -343,10 +343,9 @@ void Solver::refactor() {
   std::vector<int> basis(m);
   PDS_HIP(hipMemcpyAsync(basis.data(), basis_.p, sizeof(int) * m, hipMemcpyDeviceToHost, st_));
   sync();
-  std::vector<int> pos_P, row_R, Ridx(m, -1), Pidx(m, -1), slackpos(m, -1), slackrow(m, -1), varpos(H_.N, -1);
+  std::vector<int> pos_P, row_R, Ridx(m, -1), Pidx(m, -1), slackpos(m, -1), slackrow(m, -1);
   for (int i = 0; i < m; ++i) {
     int j = basis[i];
-    varpos[j] = i;
     if (j >= n) {
       slackpos[j - n] = i;
       slackrow[i] = j - n;
@@ -362,20 +361,12 @@ void Solver::refactor() {
     }
   const int k = (int)pos_P.size();
   if ((int)row_R.size() != k) throw SingularError("basis is structurally singular");
-  // Coupling block C = A[covered rows, basic structurals], by rows and by kernel columns.
-  std::vector<int> crp(m + 1, 0), cci, ccp(k + 1, 0), ccr;
-  std::vector<double> ccv, ccv2;
-  for (int r = 0; r < m; ++r) {
-    if (slackpos[r] >= 0)
-      for (int e = H_.rptr[r]; e < H_.rptr[r + 1]; ++e) {
-        int j = H_.cidx[e];
-        if (j < n && varpos[j] >= 0) {
-          cci.push_back(Pidx[varpos[j]]);
-          ccv.push_back(H_.rval[e]);
-        }
-      }
-    crp[r + 1] = (int)cci.size();
-  }
+  // Coupling block C = A[covered rows, basic structurals], by kernel columns (CSC) and by rows
+  // (CSR). The CSR is a counting sort of the CSC entries by row, with the columns visited in
+  // increasing variable order: each row lists its entries as a scan of A's row would, at
+  // O(nnz of the basic structurals) instead of O(nnz of the covered rows).
+  std::vector<int> crp(m + 1, 0), ccp(k + 1, 0), ccr;
+  std::vector<double> ccv2;
   for (int c = 0; c < k; ++c) {
     int j = basis[pos_P[c]];
     for (int e = H_.cptr[j]; e < H_.cptr[j + 1]; ++e) {
@@ -383,10 +374,22 @@ void Solver::refactor() {
       if (slackpos[r] >= 0) {
         ccr.push_back(r);
         ccv2.push_back(H_.cval[e]);
+        crp[r + 1]++;
       }
     }
     ccp[c + 1] = (int)ccr.size();
   }
+  for (int r = 0; r < m; ++r) crp[r + 1] += crp[r];
+  std::vector<int> cci(ccr.size()), fill(crp.begin(), crp.end() - 1), by_var(k);
+  std::vector<double> ccv(ccr.size());
+  for (int c = 0; c < k; ++c) by_var[c] = c;
+  std::sort(by_var.begin(), by_var.end(), [&](int a, int b) { return basis[pos_P[a]] < basis[pos_P[b]]; });
+  for (int c : by_var)
+    for (int q = ccp[c]; q < ccp[c + 1]; ++q) {
+      const int slot = fill[ccr[q]]++;
+      cci[slot] = c;
+      ccv[slot] = ccv2[q];
+    }
   pos_P.resize(std::max(k, 1));
   row_R.resize(std::max(k, 1));
   PDS_HIP(hipMemcpyAsync(pos_P_.p, pos_P.data(), sizeof(int) * pos_P.size(), hipMemcpyHostToDevice, st_));
