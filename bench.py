@@ -282,6 +282,10 @@ def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, s
                          "mean_iterations": round(float(it_w.mean()), 1),
                          "us_per_lp_iteration": round(1e3 * (ms_w - ms_pred) / max(1.0, float(it_w.sum())), 4),
                          "repaired_columns_mean": round(float(rep.mean()), 2)},
+        "iterations_percentiles": {
+            "slack": {q: round(float(np.percentile(it_s, v)), 1) for q, v in (("p50", 50), ("p90", 90), ("p99", 99), ("max", 100))},
+            "learned_warm": {q: round(float(np.percentile(it_w, v)), 1) for q, v in (("p50", 50), ("p90", 90), ("p99", 99), ("max", 100))},
+            "note": "the held-out LPs run concurrently (one workgroup each), so the batch time follows the slowest LP"},
         "speedup_solves_per_s": round(ms_s / ms_w, 3),
         "iteration_ratio": round(float(it_s.mean()) / max(1e-9, float(it_w.mean())), 3),
         "status_match_and_optimal": int(ok.sum()), "of": n_eval,
