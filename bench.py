@@ -224,7 +224,7 @@ def transport(cpu, S=500, D=1000, seed=2, ratio_test=0, mark=200):
     return out
 
 
-def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, seed=4, sigma=0.1):
+def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, plain_sample=20, m=256, n=640, k=5, seed=4, sigma=0.1):
     """Learned warm start vs slack start on the same GPU: an LP family of the single-LP recipe
     (k nonzeros per column) whose b and c are perturbed by lognormal(0, sigma); the network is
     trained on the bases the batched GPU solver finds for n_train instances and evaluated on
@@ -301,6 +301,30 @@ def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, m=256, n=640, k=5, s
         "tuned": {"block": prm.block, "lds_inv_max_m": prm.lds_inv_max_m, "wgs_per_cu": prm.wgs_per_cu,
                   "refactor_freq": prm.refactor_freq},
     })
+    if plain_sample > 0:
+        # The plain single-LP GPU solver (slack start, one LP at a time) on the first held-out LPs.
+        from parallel_ds_amd import DualSimplex, default_params
+        log("learned warm start: plain single-LP GPU solver on %d held-out instances" % plain_sample)
+        t0 = time.perf_counter()
+        zs, its, sts = [], [], []
+        for lp in held[:plain_sample]:
+            with DualSimplex(lp, default_params()) as s:
+                s.initDualSimplex()
+                if s.presolve("minInfeas") == 0:
+                    s.solve()
+                r = s.result()
+            zs.append(r.Z)
+            its.append(r.iterations)
+            sts.append(r.status)
+        sec = time.perf_counter() - t0
+        zs = np.array(zs)
+        agree = np.abs(zs - obj_s[:plain_sample]) <= 1e-9 * np.maximum(1.0, np.abs(obj_s[:plain_sample]))
+        out["plain_single_lp_gpu"] = {
+            "solves_per_s": round(plain_sample / sec, 3), "instances": plain_sample,
+            "mean_iterations": round(float(np.mean(its)), 1), "optimal": int(np.sum(np.array(sts) == 0)),
+            "objective_agrees_with_batched_slack": int(agree.sum()),
+            "learned_warm_batched_speedup": round(out["learned_warm"]["solves_per_s"] / (plain_sample / sec), 1),
+            "note": "wall time of load + solve per LP through the single-LP C-ABI path, one LP at a time"}
     if cpu_sample > 0:
         log("learned warm start: CPU oracle on %d instances" % cpu_sample)
         import multiprocessing as mp
