@@ -37,6 +37,9 @@ METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel e
 #   transportation LP 500 x 1000, seed 2 (all 2,294 iterations): 9,916.4 KiB x 1.99976 + 3,849.1 KiB
 SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7768498}
 TRANSPORT_SPMV_TRAFFIC_BYTES = {(500, 1000, 2): 24247818}
+#   4,096 batched packing LPs (seeds 3000+k), the timed k_batch<512> launch of 479 ms:
+#     584,897,945 KiB x 1.99976 + 906,623,171 KiB
+BATCH_TRAFFIC_BYTES = {4096: 2126109373945}
 
 
 def log(msg):
@@ -88,6 +91,20 @@ def _oracle_solve_one(args):
     return _t.perf_counter() - t0, r.status, r.Z
 
 
+def batch_roofline(ms, m_of, iters, lds_inv_max_m):
+    """HBM roofline of a batched solve: the dense B^-1 of every LP whose inverse is not kept in
+    LDS is read and written once per basis change (16 m^2 bytes per LP-iteration, the fused
+    FTRAN + DSE + product-form pass); vectors, pricing and refactors are not counted."""
+    import numpy as np
+    m = np.asarray(m_of, np.float64)
+    glob = m > lds_inv_max_m
+    byts = float(np.sum(16.0 * m[glob] ** 2 * np.asarray(iters, np.float64)[glob]))
+    ach = byts / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_batch": round(byts),
+            "per_lp_iteration": "16 m^2 (B^-1 read + write)"}
+
+
 def batched_family(count, tune_sample, reps, cpu_sample):
     """4,096 packing LPs, m ~ U{32..256}, n = 2m, density 10%, one persistent launch."""
     from parallel_ds_amd import lp as lpmod
@@ -110,7 +127,9 @@ def batched_family(count, tune_sample, reps, cpu_sample):
            "ms_per_iter_per_lp": round(ms / max(1, float(it.sum())) * count, 5),
            "tuned": {"block": best.block, "lds_inv_max_m": best.lds_inv_max_m, "wgs_per_cu": best.wgs_per_cu,
                      "refactor_freq": best.refactor_freq,
-                     "candidates": tried, "sample": tune_sample, "sample_ms": round(tune_ms, 3)}}
+                     "candidates": tried, "sample": tune_sample, "sample_ms": round(tune_ms, 3)},
+           "roofline": dict(batch_roofline(ms, [lp.m for lp in lps], it, best.lds_inv_max_m),
+                            traffic=BATCH_TRAFFIC_BYTES.get(count))}
     try:
         import json as _j
         g = _j.load(open(os.path.join(ROOT, "tests", "golden", "highs_objectives.json")))["config3"]
@@ -286,6 +305,8 @@ def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, plain_sample=20, m=2
             "slack": {q: round(float(np.percentile(it_s, v)), 1) for q, v in (("p50", 50), ("p90", 90), ("p99", 99), ("max", 100))},
             "learned_warm": {q: round(float(np.percentile(it_w, v)), 1) for q, v in (("p50", 50), ("p90", 90), ("p99", 99), ("max", 100))},
             "note": "the held-out LPs run concurrently (one workgroup each), so the batch time follows the slowest LP"},
+        "roofline": {"slack": batch_roofline(ms_s, [m] * n_eval, it_s, prm.lds_inv_max_m),
+                     "learned_warm": batch_roofline(ms_w - ms_pred, [m] * n_eval, it_w, prm.lds_inv_max_m)},
         "speedup_solves_per_s": round(ms_s / ms_w, 3),
         "iteration_ratio": round(float(it_s.mean()) / max(1e-9, float(it_w.mean())), 3),
         "status_match_and_optimal": int(ok.sum()), "of": n_eval,
