@@ -18,7 +18,7 @@ them:
   hand-computed 2 x 3 LP);
 * ``select_top_m``: the starting basis chosen from per-variable scores.
 
-Only tests/, the graft smoke check and bench.py's CPU baseline may import it.
+Only tests/, the smoke check and bench.py's CPU baseline may import it.
 """
 import numpy as np
 
