@@ -775,13 +775,11 @@ __global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, con
 // Reduced-cost update (sequentialDualSimplex.cpp:428-467, and
 // simpleReducedCostsUpate :163-172 in phase 1) with the stability check.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase, int it) {
-  DevScalars* S = I.S;
-  if (stopped(S)) return;
-  const double theta = S->theta;
-  const int q_idx = S->q_idx;
+// Grid-stride part of the dual update: d_j -= theta * alpha_r over the
+// nonbasic set, with the dual-infeasibility and bound-flip counts.
+__device__ __forceinline__ void dupdate_body(const DevModel& M, const DevIter& I, int phase, double theta, int q_idx,
+                                             int& dinf, int& flips) {
   const int nN = M.N - M.m;
-  int dinf = 0, flips = 0;
   for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
     int j = I.nonbasis[i];
     double dj = I.d[j] - theta * I.alpha_r[i];
@@ -792,22 +790,13 @@ __global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase
       if ((fabs(xj - M.l[j]) < EPS_Z && dj < -EPS_D) || (fabs(xj - M.u[j]) < EPS_Z && dj > EPS_D)) ++flips;
     }
   }
-  dinf = block_sumi<BS>(dinf);
-  flips = block_sumi<BS>(flips);
-  if (threadIdx.x == 0) {
-    I.part_i[2 * blockIdx.x] = dinf;
-    I.part_i[2 * blockIdx.x + 1] = flips;
-  }
-  if (!last_block(&S->ctr[CTR_DUPD])) return;
-  int td = 0, tf = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
-    td += I.part_i[2 * b];
-    tf += I.part_i[2 * b + 1];
-  }
-  td = block_sumi<BS>(td);
-  tf = block_sumi<BS>(tf);
-  if (threadIdx.x != 0) return;
-  S->ctr[CTR_DUPD] = 0;
+}
+
+// Thread 0 of the last workgroup: entering/leaving duals, the stability check
+// of alpha_q[p] against alpha_r[q], bound flips, objective.
+__device__ __forceinline__ void dupdate_final(const DevModel& M, const DevIter& I, int phase, int it, double theta,
+                                              int td, int tf) {
+  DevScalars* S = I.S;
   const int p = S->p, q = S->q;
   I.d[p] = -theta;
   I.d[q] = 0.0;
@@ -826,40 +815,81 @@ __global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase
   S->alpha_p = aqp;
 }
 
+__global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double theta = S->theta;
+  int dinf = 0, flips = 0;
+  dupdate_body(M, I, phase, theta, S->q_idx, dinf, flips);
+  dinf = block_sumi<BS>(dinf);
+  flips = block_sumi<BS>(flips);
+  if (threadIdx.x == 0) {
+    I.part_i[2 * blockIdx.x] = dinf;
+    I.part_i[2 * blockIdx.x + 1] = flips;
+  }
+  if (!last_block(&S->ctr[CTR_DUPD])) return;
+  int td = 0, tf = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    td += I.part_i[2 * b];
+    tf += I.part_i[2 * b + 1];
+  }
+  td = block_sumi<BS>(td);
+  tf = block_sumi<BS>(tf);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_DUPD] = 0;
+  dupdate_final(M, I, phase, it, theta, td, tf);
+}
+
 // ---------------------------------------------------------------------------
 // Basis change: x_B (or f) update, new eta, DSE weights, swap
 // (sequentialDualSimplex.cpp:174-227).  The finaliser appends the eta's
 // pivot and its row/column of the E table.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_update(DevModel M, DevIter I, int phase, int it) {
-  DevScalars* S = I.S;
-  if (stopped(S)) return;
-  const int p_idx = S->p_idx;
-  const double ap = I.alpha_q[p_idx];
-  const double bp = I.beta[p_idx];
-  const double thP = (phase == 2 ? S->delta : S->f_p) / ap;
-  const int t = S->t;
-  double* eta = I.etas + (size_t)t * M.m;
+struct UpdateArgs {
+  int p_idx, t;
+  double ap, bp, thP;
+  double* eta;
+};
+__device__ __forceinline__ UpdateArgs update_args(const DevModel& M, const DevIter& I, int phase) {
+  const DevScalars* S = I.S;
+  UpdateArgs u;
+  u.p_idx = S->p_idx;
+  u.ap = I.alpha_q[u.p_idx];
+  u.bp = I.beta[u.p_idx];
+  u.thP = (phase == 2 ? S->delta : S->f_p) / u.ap;
+  u.t = S->t;
+  u.eta = I.etas + (size_t)u.t * M.m;
+  return u;
+}
+
+// Grid-stride part of the basis change: x_B (or f), the new eta, DSE weights.
+__device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
     double ai = I.alpha_q[i];
     if (phase == 2) {
       int j = I.basis[i];
-      I.x[j] = I.x[j] - thP * ai;
-    } else if (i != p_idx) {
-      I.f[i] = I.f[i] - ai * thP;
+      I.x[j] = I.x[j] - u.thP * ai;
+    } else if (i != u.p_idx) {
+      I.f[i] = I.f[i] - ai * u.thP;
     }
-    if (i != p_idx) {
-      eta[i] = -ai / ap;
-      double r = ai / ap;
-      I.beta[i] = I.beta[i] - 2 * ai / ap * I.tau[i] + (r * r) * bp;
+    if (i != u.p_idx) {
+      u.eta[i] = -ai / u.ap;
+      double r = ai / u.ap;
+      I.beta[i] = I.beta[i] - 2 * ai / u.ap * I.tau[i] + (r * r) * u.bp;
     } else {
-      eta[i] = 1 / ap;
+      u.eta[i] = 1 / u.ap;
     }
   }
-  if (!last_block(&S->ctr[CTR_UPD])) return;
-  // New E entries: row t (eta_t at the older pivots) and column t (older etas
-  // at the new pivot); then row t of MF = (I - L)^{-1}, where L[t][i] =
-  // E[i][t] for i at or after the previous eta on the same position.
+}
+
+// Last workgroup (every thread): new E entries, row t (eta_t at the older
+// pivots) and column t (older etas at the new pivot); then row t of
+// MF = (I - L)^{-1}, where L[t][i] = E[i][t] for i at or after the previous
+// eta on the same position; then (thread 0) the eta bookkeeping and the swap.
+__device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u) {
+  DevScalars* S = I.S;
+  const int t = u.t, p_idx = u.p_idx;
+  const double* eta = u.eta;
   const int cap = I.eta_cap;
   __shared__ double ecol[ETA_MAX];
   __shared__ int s_last;
@@ -880,30 +910,74 @@ __global__ __launch_bounds__(BS) void k_update(DevModel M, DevIter I, int phase,
     I.MF[(size_t)t * cap + j] = acc;
   }
   if (threadIdx.x != 0) return;
-  S->ctr[CTR_UPD] = 0;
   I.MF[(size_t)t * cap + t] = 1.0;
   I.eta_last[t] = lastt;
   I.E[(size_t)t * cap + t] = eta[p_idx];
   I.eta_pos[t] = p_idx;
   S->t = t + 1;
-  I.beta[p_idx] = bp / (ap * ap);
+  I.beta[p_idx] = u.bp / (u.ap * u.ap);
   const int p = S->p, q = S->q, q_idx = S->q_idx;
   if (phase == 2) {
-    I.x[q] = I.x[q] + thP;
+    I.x[q] = I.x[q] + u.thP;
   } else {
-    I.f[p_idx] = I.f[p_idx] / ap + S->corr;
+    I.f[p_idx] = I.f[p_idx] / u.ap + S->corr;
   }
   I.basis[p_idx] = q;
   I.nonbasis[q_idx] = p;
   I.posN[q] = -1;
   I.posN[p] = q_idx;
-  S->theta_P = thP;
+  S->theta_P = u.thP;
   const double theta = S->theta;
   if (phase == 2) {
     if (fabs(theta) < EPS_A) S->cycle = S->cycle + 1;
   } else {
     S->cycle = (fabs(theta) < EPS_A) ? S->cycle + 1 : 0;
   }
+}
+
+__global__ __launch_bounds__(BS) void k_update(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const UpdateArgs u = update_args(M, I, phase);
+  update_body(M, I, phase, u);
+  if (!last_block(&S->ctr[CTR_UPD])) return;
+  if (threadIdx.x == 0) S->ctr[CTR_UPD] = 0;
+  update_final(M, I, phase, u);
+}
+
+// Dual update and basis change in one launch. The basis change only waits for
+// the dual update's stability check (alpha_q[p] against alpha_r[q]), which
+// every workgroup evaluates at entry from the same two values; when it fails
+// the basis change is left out and the refactor request is raised, as the
+// separate kernels do. The DSE vector tau must be ready before the launch.
+__global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, int phase, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double theta = S->theta;
+  const UpdateArgs u = update_args(M, I, phase);
+  const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
+  int dinf = 0, flips = 0;
+  dupdate_body(M, I, phase, theta, S->q_idx, dinf, flips);
+  if (stable) update_body(M, I, phase, u);
+  dinf = block_sumi<BS>(dinf);
+  flips = block_sumi<BS>(flips);
+  if (threadIdx.x == 0) {
+    I.part_i[2 * blockIdx.x] = dinf;
+    I.part_i[2 * blockIdx.x + 1] = flips;
+  }
+  if (!last_block(&S->ctr[CTR_DUPD])) return;
+  int td = 0, tf = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+    td += I.part_i[2 * b];
+    tf += I.part_i[2 * b + 1];
+  }
+  td = block_sumi<BS>(td);
+  tf = block_sumi<BS>(tf);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_DUPD] = 0;
+    dupdate_final(M, I, phase, it, theta, td, tf);
+  }
+  if (stable) update_final(M, I, phase, u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1169,6 +1243,10 @@ void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phas
 }
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
   k_update<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
+}
+void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
+  const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
+  k_dupdate_update<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
 }
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I) {
   k_set_primal<<<nblk(M.N - M.m, BS), BS, 0, s>>>(M, I);

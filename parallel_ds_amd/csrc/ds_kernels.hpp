@@ -22,6 +22,7 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
                         double* y, int t);
 void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
+void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I);
 void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN);
 void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,

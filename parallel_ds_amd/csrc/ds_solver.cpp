@@ -576,8 +576,9 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   if (timed) ev_iter_[ev_used_++] = it;
   launch_chuzc(st_, m, i, phase, P_.ratio_test, it);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
-  launch_dupdate(st_, m, i, phase, it);
-  enqueue_stageB(phase, it, t);
+  // tau = B^-1 rho first: the dual update and the basis change then go in one launch.
+  launch_ftran_dense(st_, m, i, f, rho_.p, tau_.p, t, 0, 1);
+  launch_dupdate_update(st_, m, i, phase, it);
 }
 
 void Solver::enqueue_stageB(int phase, int it, int t) {
