@@ -68,7 +68,7 @@ typedef struct {
   int chunk;              /* iterations queued per host synchronisation (>= 1) */
   int verbose;
   int mark_iteration;     /* > 0: synchronise after this iteration and record the time (ds_result.mark_ms) */
-  int time_kernels;       /* 1: time every pivot-row SpMV launch after the mark with HIP events */
+  int time_kernels;       /* k > 0: time the pivot-row SpMV of every k-th iteration after the mark with HIP events */
 } ds_params;
 
 typedef struct {

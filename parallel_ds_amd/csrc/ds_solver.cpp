@@ -570,7 +570,8 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
   // Timed launches carry a start/stop event pair that records the kernel's own
   // execution interval (hipExtLaunchKernelGGL).
-  const bool timed = P_.time_kernels && marked_ && ev_used_ < (int)ev_iter_.size();
+  // Every time_kernels-th iteration: a per-launch event pair costs the loop a few percent.
+  const bool timed = P_.time_kernels > 0 && marked_ && it % P_.time_kernels == 0 && ev_used_ < (int)ev_iter_.size();
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1, timed ? ev_[2 * ev_used_] : nullptr,
               timed ? ev_[2 * ev_used_ + 1] : nullptr);
   if (timed) ev_iter_[ev_used_++] = it;
