@@ -662,6 +662,8 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   const bool live = i < M.m;
   const int c = live ? F.Pidx_of_pos[i] : 0;
   const int r = (live && c < 0) ? F.slackrow_of_pos[i] : 0;
+  const int qb = (live && c < 0) ? F.crp[r] : 0;
+  const int qe = (live && c < 0) ? F.crp[r + 1] : 0;
   const int cb = M.cptr[j];
   const int ne = min(M.cptr[j + 1] - cb, FT_MAXCOL);
   for (int e = threadIdx.x; e < ne; e += BS) {
@@ -682,13 +684,30 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
     double ar = 0.0;
     for (int e = 0; e < ne; ++e)
       if (er[e] == r) ar = ev[e];
+    // Coupling entries four at a time, so their loads overlap; each x_P value
+    // and the sum over q keep the order of a one-at-a-time loop.
+    constexpr int U = 4;
     double s = 0.0;
-    for (int q = F.crp[r]; q < F.crp[r + 1]; ++q) {
-      int c2 = F.cci[q];
-      double xp = 0.0;
+    for (int q0 = qb; q0 < qe; q0 += U) {
+      int c2[U];
+      double cv[U], xp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = q0 + u < qe;
+        c2[u] = ok ? F.cci[q0 + u] : 0;
+        cv[u] = ok ? F.ccv[q0 + u] : 0.0;
+        xp[u] = 0.0;
+      }
       for (int e = 0; e < ne; ++e)
-        if (eri[e] >= 0) xp += ev[e] * F.KinvT[(size_t)eri[e] * k + c2];
-      s += F.ccv[q] * xp;
+        if (eri[e] >= 0) {
+          const double* row = F.KinvT + (size_t)eri[e] * k;
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (q0 + u < qe) xp[u] += ev[e] * row[c2[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + u < qe) s += cv[u] * xp[u];
     }
     x = ar - s;
   }
