@@ -46,6 +46,7 @@ struct GJWork {
   int* nA;
   int* nB;
   int use_mfma;   // trailing update on FP64 MFMA (1) or plain FMA (0)
+  double* rowscale;  // k: largest |entry| of each row of K, kept in step with the row swaps
 };
 constexpr int GJ_NB = 64;
 void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, const DevFactor& F, double* W);

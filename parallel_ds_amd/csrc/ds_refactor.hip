@@ -47,13 +47,11 @@ __global__ __launch_bounds__(256) void k_build_kernel(DevModel M, const int* bas
 // row.  Only the winners are used (the panel itself is eliminated by
 // k_mid_gemm / k_gemm_sub with Pinv), so the multipliers use the pivot's
 // reciprocal and pivoted rows are not kept.
-constexpr int TS_ROWS = 256;
-constexpr int TS_THREADS = TS_ROWS;
-constexpr int TS_WAVES = TS_THREADS / WAVE;
+constexpr int TS_ROWS = 256;  // rows per group of the multi-round tournament (4 waves)
 constexpr int TS_BLK = 16;
-constexpr int GJ_COLS = 2 * NB / TS_WAVES;  // [B | I] columns per wave in the final Gauss-Jordan
+constexpr double SING_REL = 1e-13;  // relative pivot below which the kernel counts as singular
 static_assert(NB == WAVE, "one pivot-block row per lane");
-static_assert(NB % TS_BLK == 0 && NB % GJ_COLS == 0, "unrolled blocks tile the panel");
+static_assert(NB % TS_BLK == 0, "unrolled blocks tile the panel");
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ double uniform_d(double v) {
@@ -88,17 +86,29 @@ __device__ __forceinline__ double recip_nr(double p) {
   return fma(r, fma(-p, r, 1.0), r);
 }
 
-__global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int j, int nb, const int* in_rows,
+// WV waves, 64 WV rows per group. WV = 4 stages the rows in an LDS tile. WV = 8
+// (one group of up to 512 rows, a single round) loads them directly and
+// reads the winners' block for the Gauss-Jordan from W.
+template <int WV>
+__global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int j, int nb, const int* in_rows,
                                                      const int* in_n, int in_groups, int* out_rows, int* out_n,
-                                                     int final_, double* Pinv, int* piv, int* flag) {
-  __shared__ int rid[TS_ROWS];
+                                                     int final_, double* Pinv, int* piv, int* flag,
+                                                     const double* rowscale) {
+  constexpr int TS_WAVES = WV;
+  constexpr int TS_THREADS = WV * WAVE;
+  constexpr int ROWS = WV * WAVE;
+  constexpr int GJ_COLS = 2 * NB / TS_WAVES;  // [B | I] columns per wave in the final Gauss-Jordan
+  constexpr bool TILE = WV == 4;
+  static_assert(NB % GJ_COLS == 0, "unrolled blocks tile the panel");
+  __shared__ int rid[ROWS];
   __shared__ int goff[5];
   __shared__ double cand[2][TS_WAVES][NB];
   __shared__ double cv[2][TS_WAVES];
   __shared__ int ci[2][TS_WAVES];
+  __shared__ double cs[2][TS_WAVES];  // the candidate row's scale (largest |entry| of its panel row, as loaded)
   __shared__ int win[NB];
   __shared__ int wrow[NB];
-  __shared__ double tile[TS_ROWS][NB + 1];  // the group's rows of the panel, as loaded
+  __shared__ double tile[TILE ? ROWS : 1][NB + 1];  // the group's rows of the panel, as loaded
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
   const int wv = uniform_i(tid / WAVE), lane = tid % WAVE;
@@ -114,8 +124,8 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
   __syncthreads();
   int R;
   if (!in_rows) {
-    const int r0 = j + w * TS_ROWS;
-    R = min(k, r0 + TS_ROWS) - r0;
+    const int r0 = j + w * ROWS;
+    R = min(k, r0 + ROWS) - r0;
     for (int t = tid; t < R; t += TS_THREADS) rid[t] = r0 + t;
   } else {
     R = goff[4];
@@ -127,14 +137,24 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
   }
   R = uniform_i(R);
   __syncthreads();
-  // Coalesced load (lane = column) into LDS, then lane = row into registers.
-#pragma unroll 8
-  for (int r = wv; r < R; r += TS_WAVES) tile[r][lane] = lane < nb ? W[(size_t)rid[r] * k + j + lane] : 0.0;
-  __syncthreads();
   bool live = tid < R;
   double a[NB];
+  if constexpr (TILE) {
+    // Coalesced load (lane = column) into LDS, then lane = row into registers.
+#pragma unroll 8
+    for (int r = wv; r < R; r += TS_WAVES) tile[r][lane] = lane < nb ? W[(size_t)rid[r] * k + j + lane] : 0.0;
+    __syncthreads();
 #pragma unroll
-  for (int x = 0; x < NB; ++x) a[x] = live ? tile[tid][x] : 0.0;
+    for (int x = 0; x < NB; ++x) a[x] = live ? tile[tid][x] : 0.0;
+  } else {
+    const double* src = W + (size_t)(live ? rid[tid] : 0) * k + j;
+#pragma unroll
+    for (int x = 0; x < NB; ++x) a[x] = (live && x < nb) ? src[x] : 0.0;
+  }
+  // Row scale for the final round's singularity test: the row's largest
+  // |entry| in K before any elimination (a dependent row shrinks to rounding
+  // level in every later panel).
+  const double rmax = live ? rowscale[rid[tid]] : 0.0;
   const int steps = min(nb, R);
   for (int cb = 0; cb * TS_BLK < steps; ++cb) {
 #pragma unroll
@@ -153,9 +173,11 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
 #pragma unroll
           for (int x = s; x < NB; ++x) cand[b][wv][x] = a[x];
         }
+        const double crow = cl >= 0 ? readlane_d(rmax, cl) : 0.0;
         if (lane == 0) {
           cv[b][wv] = m;
           ci[b][wv] = cl >= 0 ? wv * WAVE + cl : INT_MAX;
+          cs[b][wv] = crow;
         }
         __syncthreads();
         double gv = -2.0;
@@ -173,6 +195,9 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
         // steps <= R, so some row is live and gr is a row of this group.
         if (tid == 0) win[st] = gr;
         if (tid == gr) live = false;
+        // Final round: a pivot at or below SING_REL of its row's scale in K marks
+        // the kernel singular (rounding leaves dependent rows near, not at, zero).
+        if (final_ && tid == 0 && gv > 0.0 && gv <= SING_REL * cs[b][gr / WAVE]) *flag = 1;
         if (gv > 0.0) {
           const double* pr = cand[b][gr / WAVE];
           const double l = a[s] * recip_nr(pr[s]);
@@ -212,7 +237,10 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
       const int c = GJ_COLS * wv + x;
       double v = 0.0;
       if (lane < steps) {
-        if (c < NB) v = tile[sl][c];
+        if (c < NB) {
+          if constexpr (TILE) v = tile[sl][c];
+          else v = c < nb ? W[(size_t)rid[sl] * k + j + c] : 0.0;
+        }
         else v = (c - NB == lane) ? 1.0 : 0.0;
       }
       g[x] = v;
@@ -269,7 +297,8 @@ __global__ __launch_bounds__(TS_THREADS) void k_tslu(const double* W, int k, int
 }
 
 // Apply the panel's row swaps to every column.
-__global__ __launch_bounds__(256) void k_swap_rows(double* W, int k, int j, int nb, const int* piv) {
+__global__ __launch_bounds__(256) void k_swap_rows(double* W, int k, int j, int nb, const int* piv,
+                                                   double* rowscale) {
   int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= k) return;
   for (int s = 0; s < nb; ++s) {
@@ -278,8 +307,24 @@ __global__ __launch_bounds__(256) void k_swap_rows(double* W, int k, int j, int 
       double a = W[(size_t)(j + s) * k + c];
       W[(size_t)(j + s) * k + c] = W[(size_t)r * k + c];
       W[(size_t)r * k + c] = a;
+      if (c == 0) {
+        double g = rowscale[j + s];
+        rowscale[j + s] = rowscale[r];
+        rowscale[r] = g;
+      }
     }
   }
+}
+
+// rowscale[r] = max_c |W[r][c]| (one wave per row).
+__global__ __launch_bounds__(256) void k_row_scale(const double* W, int k, double* rowscale) {
+  const int r = (blockIdx.x * 256 + threadIdx.x) / WAVE;
+  const int lane = threadIdx.x % WAVE;
+  if (r >= k) return;
+  double v = 0.0;
+  for (int c = lane; c < k; c += WAVE) v = fmax(v, fabs(W[(size_t)r * k + c]));
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  if (lane == 0) rowscale[r] = v;
 }
 
 // Save the (swapped) panel columns and reset them to identity columns.
@@ -433,19 +478,27 @@ void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, con
 
 void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
   if (k <= 0) return;
+  k_row_scale<<<(int)(((long)k * WAVE + 255) / 256), 256, 0, s>>>(G.W, k, G.rowscale);
   for (int j = 0; j < k; j += NB) {
     int nb = (k - j < NB) ? (k - j) : NB;
     int groups = (k - j + TS_ROWS - 1) / TS_ROWS;
     int* in_rows = G.rowsA;
     int* in_n = G.nA;
-    k_tslu<<<groups, TS_THREADS, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, groups == 1 ? 1 : 0,
-                                         G.Pinv, G.piv, G.flag);
+    if (groups == 2) {
+      // 257..512 rows: one 8-wave group in a single round instead of two groups and a merge.
+      k_tslu<8><<<1, 8 * WAVE, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, 1, G.Pinv, G.piv, G.flag,
+                                        G.rowscale);
+      groups = 1;
+    } else {
+      k_tslu<4><<<groups, 4 * WAVE, 0, s>>>(G.W, k, j, nb, nullptr, nullptr, 0, in_rows, in_n, groups == 1 ? 1 : 0,
+                                            G.Pinv, G.piv, G.flag, G.rowscale);
+    }
     int* out_rows = G.rowsB;
     int* out_n = G.nB;
     while (groups > 1) {
       int g2 = (groups + 3) / 4;
-      k_tslu<<<g2, TS_THREADS, 0, s>>>(G.W, k, j, nb, in_rows, in_n, groups, out_rows, out_n, g2 == 1 ? 1 : 0,
-                                       G.Pinv, G.piv, G.flag);
+      k_tslu<4><<<g2, 4 * WAVE, 0, s>>>(G.W, k, j, nb, in_rows, in_n, groups, out_rows, out_n, g2 == 1 ? 1 : 0,
+                                        G.Pinv, G.piv, G.flag, G.rowscale);
       int* tr = in_rows;
       in_rows = out_rows;
       out_rows = tr;
@@ -454,7 +507,7 @@ void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
       out_n = tn;
       groups = g2;
     }
-    k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv);
+    k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv, G.rowscale);
     k_panel_save<<<(int)(((long)k * nb + 255) / 256), 256, 0, s>>>(G.W, k, j, nb, G.Pb);
     k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv);
     dim3 grid((k + 63) / 64, (k + 63) / 64);

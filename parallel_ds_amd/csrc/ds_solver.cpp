@@ -313,6 +313,7 @@ void Solver::load(const HostModel& hm) {
   ccr_.alloc(nnz);
   ccv2_.alloc(nnz);
   piv_.alloc(m);
+  rowscale_.alloc(m);
   perm_.alloc(m);
   flag_.alloc(1);
   Pb_.alloc((size_t)m * GJ_NB);
@@ -414,7 +415,7 @@ void Solver::refactor() {
     int zero = 0;
     PDS_HIP(hipMemcpyAsync(flag_.p, &zero, sizeof(int), hipMemcpyHostToDevice, st_));
     launch_build_kernel(st_, M(), basis_.p, F(), GW_.p);
-    GJWork G{GW_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_};
+    GJWork G{GW_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_, rowscale_.p};
     launch_gj_panels(st_, k, G);
     int flag = 0;
     std::vector<int> piv(k), perm(k);

@@ -113,6 +113,7 @@ class Solver {
   DBuf<DevScalars> S_;
   DevScalars* hS_ = nullptr;  // pinned mirror
   // factor
+  DBuf<double> rowscale_;
   DBuf<double> Kinv_, KinvT_, GW_, Pb_, Pinv_, ccv_, ccv2_;
   DBuf<int> pos_P_, row_R_, Ridx_, Pidx_, slackpos_, slackrow_, crp_, cci_, ccp_, ccr_, piv_, perm_, flag_, rowsA_,
       rowsB_, nA_, nB_;

@@ -180,3 +180,37 @@ def test_time_pivot_row_ext(gpu):
         w2, c2, _ = s.time_pivot_row_ext(np.ones(L.m), reps=3, flush_bytes=0)
     assert w > 0 and c > 0 and w2 > 0 and c2 == 0
     assert by == 12.0 * L.nnz + 4.0 * (L.n + 1) + 8.0 * L.n + 8.0 * L.m
+
+
+@pytest.mark.parametrize("dependent", [False, True])
+def test_refactor_flags_a_numerically_singular_kernel(gpu, dependent):
+    """A 300 x 300 basis kernel (two tournament rounds in the first panels) with one column an
+    exact combination of two others. Rounding leaves its pivot near 1e-16 of the row's scale, not
+    at zero; the refactorization must still report it singular (DS_ESINGULAR). Without the
+    dependency the same basis factors and FTRAN matches a dense solve."""
+    from parallel_ds_amd._native import DsError
+    rng = np.random.default_rng(5)
+    m, n, k = 320, 400, 300
+    A = (rng.random((m, n)) < 0.05) * rng.uniform(0.1, 1.0, (m, n))
+    A[np.arange(m), np.arange(m) % n] += rng.uniform(0.5, 1.0, m)
+    if dependent:
+        A[:, 2] = A[:, 0] + 0.7 * A[:, 1]
+    import scipy.sparse as sp
+    C = sp.csc_matrix(A)
+    C.sort_indices()
+    L = lpmod.packing_lp(m, n, C.indptr.astype(np.int32), C.indices.astype(np.int32), C.data.copy(),
+                         A @ np.ones(n) + 1.0, -np.ones(n))
+    basis = np.concatenate([np.arange(k), n + np.arange(k, m)]).astype(np.int32)
+    with DualSimplex(L, CAP) as s:
+        s.initDualSimplex()
+        if dependent:
+            with pytest.raises(DsError) as e:
+                s.set_basis(basis)
+            assert e.value.code == -3
+        else:
+            s.set_basis(basis)
+            B = np.hstack([A, np.eye(m)])[:, basis]
+            b = rng.standard_normal(m)
+            x = s.ftran(b)
+            ref = np.linalg.solve(B, b)
+            assert np.abs(x - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
