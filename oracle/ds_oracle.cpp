@@ -259,6 +259,32 @@ struct Eta {
 };
 
 /* linalg::PFIsolve (linalg.cpp:12-46). */
+/* Harris two-pass selection over candidate positions F (ascending): pass 1
+ * bounds the step by the EPS_D-relaxed ratios, pass 2 takes the largest
+ * |alpha| among the candidates whose plain ratio is within the bound; the
+ * strict comparison keeps the lowest position on ties. */
+template <typename DOf>
+int harris_pick(int nF, const int* F, DOf dof, const double* alpha) {
+  double tmax = BIG;
+  for (int s = 0; s < nF; ++s) {
+    const int i = F[s];
+    const double dj = dof(i);
+    const double t = alpha[i] > 0 ? (dj + EPS_D) / alpha[i] : (dj - EPS_D) / alpha[i];
+    tmax = std::min(tmax, t);
+  }
+  int q_idx = -1;
+  double amax = -1.0;
+  for (int s = 0; s < nF; ++s) {
+    const int i = F[s];
+    const double t = dof(i) / alpha[i];
+    if (t <= tmax && std::fabs(alpha[i]) > amax) {
+      amax = std::fabs(alpha[i]);
+      q_idx = i;
+    }
+  }
+  return q_idx;
+}
+
 void pfi_apply(const std::vector<Eta>& etas, std::vector<double>& sol, bool transpose) {
   int m = (int)sol.size();
   if (!transpose) {
@@ -292,6 +318,7 @@ struct Solver {
   int status = ORA_OPTIMAL;
   bool solved = false;
   int refactors = 0, phase1_restarts = 0;
+  int bfrt_exhausted = 0; /* BFRT loops that flipped every candidate */
   SplitMix rng;
 
   Solver(const Model& mod, const ora_params& prm)
@@ -576,22 +603,7 @@ struct Solver {
   /* Harris two-pass (the method behind tpp:576-611): pass 1 bounds the step
    * with the EPS_D-relaxed ratios, pass 2 takes the largest |alpha| under it. */
   int harrisTest(const std::vector<int>& Fs, const std::vector<double>& alpha) const {
-    double tmax = BIG;
-    for (int i : Fs) {
-      double dj = d[nonbasis[i]];
-      double t = alpha[i] > 0 ? (dj + EPS_D) / alpha[i] : (dj - EPS_D) / alpha[i];
-      tmax = std::min(tmax, t);
-    }
-    int q_idx = -1;
-    double amax = -1.0;
-    for (int i : Fs) {
-      double t = d[nonbasis[i]] / alpha[i];
-      if (t <= tmax && std::fabs(alpha[i]) > amax) {
-        amax = std::fabs(alpha[i]);
-        q_idx = i;
-      }
-    }
-    return q_idx;
+    return harris_pick((int)Fs.size(), Fs.data(), [&](int i) { return d[nonbasis[i]]; }, alpha.data());
   }
 
   int chooseColumn(const std::vector<int>& Fs, const std::vector<double>& alpha) const {
@@ -816,6 +828,11 @@ struct Solver {
         }
         if (cond) Fs.push_back(i);
       }
+      /* A Free candidate enters F with |alpha| <= EPS_A (seq.cpp:361-365); when alpha is 0 its
+       * ratio is infinite and simpleRatioTest returns -1, which the reference then uses as an
+       * index (undefined behaviour).  Here a candidate set without a finite ratio is treated
+       * as an empty one, as the GPU ratio test does. */
+      if (!Fs.empty() && chooseColumn(Fs, alpha) < 0) Fs.clear();
       if (Fs.empty()) {
         if (unbound_checked) {
           solved = false;
@@ -831,14 +848,21 @@ struct Solver {
       }
       unbound_checked = false;
       int q_idx = -1, q = -1;
+      /* When every candidate flips, the loop ends with F empty and q still the last flipped
+       * candidate, which then enters the basis without its flip: the row's remaining
+       * infeasibility is left on the leaving variable (sequentialDualSimplex.cpp:398-422).
+       * Restated as is; counted in bfrt_exhausted. */
       while (!Fs.empty()) {
-        q_idx = chooseColumn(Fs, alpha);
+        const int qi = chooseColumn(Fs, alpha);
+        if (qi < 0) break; /* only infinite ratios left: keep the last flip candidate */
+        q_idx = qi;
         q = nonbasis[q_idx];
         if (M.bt[q] == ORA_BOXED) {
           double span = M.u[q] - M.l[q];
           if (delta - span * std::fabs(alpha[q_idx]) <= 0) break;
           delta -= span * std::fabs(alpha[q_idx]);
           Fs.erase(std::remove(Fs.begin(), Fs.end(), q_idx), Fs.end());
+          if (Fs.empty()) bfrt_exhausted++;
         } else {
           break;
         }
@@ -967,6 +991,7 @@ extern "C" int ora_solve_from(int m, int n, const int* col_ptr, const int* row_i
     result->phase1_restarts = S.phase1_restarts;
     result->Z = Z;
     result->phase1_iterations = S.phase1_iterations;
+    result->bfrt_exhausted = S.bfrt_exhausted;
     if (x_out)
       for (int j = 0; j < M.N; ++j) x_out[j] = S.x[j];
     if (basis_out)
@@ -1088,4 +1113,8 @@ extern "C" int ora_simple_ratio_test(int nF, const int* F, const double* d_of_po
     }
   }
   return q_idx;
+}
+
+extern "C" int ora_harris_ratio_test(int nF, const int* F, const double* d_of_pos, const double* alpha) {
+  return harris_pick(nF, F, [&](int i) { return d_of_pos[i]; }, alpha);
 }

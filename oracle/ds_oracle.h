@@ -16,7 +16,13 @@
  *     into an ITERATION_LIMIT status (MAX_ITER is defined but unused in the
  *     reference, baseDualSimplex.hpp:45);
  *   - Free rows get a Free slack (the reference pushes no type,
- *     problem.tpp:201-218).
+ *     problem.tpp:201-218);
+ *   - a perturbation of exactly 0 (magnitude 100*EPS_D + PSI*c_j = 0, i.e.
+ *     c_j = -1) is left at 0: the reference's clamping loop
+ *     (baseDualSimplex.tpp:262-266) never ends on it;
+ *   - a ratio-test candidate set whose ratios are all infinite (a Free
+ *     candidate with alpha = 0, sequentialDualSimplex.cpp:361-365) is treated
+ *     as empty; the reference indexes with the -1 that simpleRatioTest returns.
  */
 #pragma once
 #include <stdint.h>
@@ -53,6 +59,8 @@ typedef struct {
   int phase1_restarts;
   double Z;                /* objective with the offset removed */
   int phase1_iterations;   /* iterations of the initial phase 1 */
+  int bfrt_exhausted;      /* bound-flipping loops that flipped every candidate (the last one
+                              then enters unflipped, sequentialDualSimplex.cpp:398-422) */
 } ora_result;
 
 /* Whole solve.  A is m x n (structurals only) in CSC.  Row types/ranges follow
@@ -92,6 +100,12 @@ int ora_pfi_solve(int m, int t, const double* etas, const int* piv, const double
  * positions F; returns q_idx or -1. */
 int ora_simple_ratio_test(int nF, const int* F, const double* d_of_pos, const double* alpha,
                           int tie_rule);
+
+/* Harris two-pass ratio test (ratio_test = 1) over candidate positions F:
+ * pass 1 bounds the step by the EPS_D-relaxed ratios, pass 2 takes the largest
+ * |alpha| among the candidates whose plain ratio is within that bound (lowest
+ * position on ties); returns q_idx or -1. */
+int ora_harris_ratio_test(int nF, const int* F, const double* d_of_pos, const double* alpha);
 
 #ifdef __cplusplus
 }

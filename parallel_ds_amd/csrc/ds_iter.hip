@@ -901,6 +901,14 @@ __device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I,
   }
 }
 
+// Entry i of the new eta, -alpha_q[i]/alpha_p off the pivot and 1/alpha_p at
+// it: the value update_body writes, recomputed from alpha_q (written by an
+// earlier launch) so that the last workgroup never reads eta entries that
+// other workgroups of the same launch wrote.
+__device__ __forceinline__ double eta_entry(const DevIter& I, const UpdateArgs& u, int i) {
+  return (i != u.p_idx) ? -I.alpha_q[i] / u.ap : 1 / u.ap;
+}
+
 // Last workgroup (every thread): new E entries, row t (eta_t at the older
 // pivots) and column t (older etas at the new pivot); then row t of
 // MF = (I - L)^{-1}, where L[t][i] = E[i][t] for i at or after the previous
@@ -908,14 +916,13 @@ __device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I,
 __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u) {
   DevScalars* S = I.S;
   const int t = u.t, p_idx = u.p_idx;
-  const double* eta = u.eta;
   const int cap = I.eta_cap;
   __shared__ double ecol[ETA_MAX];
   __shared__ int s_last;
   if (threadIdx.x == 0) s_last = -1;
   __syncthreads();
   for (int jj = threadIdx.x; jj < t; jj += BS) {
-    I.E[(size_t)t * cap + jj] = eta[I.eta_pos[jj]];
+    I.E[(size_t)t * cap + jj] = eta_entry(I, u, I.eta_pos[jj]);
     double e = I.etas[(size_t)jj * M.m + p_idx];
     I.E[(size_t)jj * cap + t] = e;
     ecol[jj] = e;
@@ -931,7 +938,7 @@ __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I
   if (threadIdx.x != 0) return;
   I.MF[(size_t)t * cap + t] = 1.0;
   I.eta_last[t] = lastt;
-  I.E[(size_t)t * cap + t] = eta[p_idx];
+  I.E[(size_t)t * cap + t] = eta_entry(I, u, p_idx);
   I.eta_pos[t] = p_idx;
   S->t = t + 1;
   I.beta[p_idx] = u.bp / (u.ap * u.ap);

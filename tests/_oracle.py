@@ -19,7 +19,7 @@ class OraParams(C.Structure):
 class OraResult(C.Structure):
     _fields_ = [("status", C.c_int), ("solved", C.c_int), ("iterations", C.c_int), ("perturbed", C.c_int),
                 ("refactors", C.c_int), ("phase1_restarts", C.c_int), ("Z", C.c_double),
-                ("phase1_iterations", C.c_int)]
+                ("phase1_iterations", C.c_int), ("bfrt_exhausted", C.c_int)]
 
 
 _lib = None
@@ -123,3 +123,13 @@ def ratio_test(F, d_of_pos, alpha, tie_rule=0):
     d_of_pos = np.ascontiguousarray(d_of_pos, np.float64)
     alpha = np.ascontiguousarray(alpha, np.float64)
     return lib().ora_simple_ratio_test(len(F), P(F), P(d_of_pos), P(alpha), tie_rule)
+
+
+def harris_ratio_test(F, d_of_pos, alpha):
+    F = np.ascontiguousarray(F, np.int32)
+    d_of_pos = np.ascontiguousarray(d_of_pos, np.float64)
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    L = lib()
+    L.ora_harris_ratio_test.restype = C.c_int
+    L.ora_harris_ratio_test.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    return L.ora_harris_ratio_test(len(F), P(F), P(d_of_pos), P(alpha))
