@@ -22,6 +22,7 @@
  *   ds_btran          BTran -> solveLinSys(e_p, rho, true)  cpu/sequentialDualSimplex.cpp:89-92
  *   ds_ftran          solveLinSys(b, x, false)          cpu/sequentialDualSimplex.cpp:28-48
  *   ds_ftran_column   FTran(q, alpha_q)                 cpu/sequentialDualSimplex.cpp:107-110
+ *   ds_chuzr          DSEPricing + checkVarIsInfeas     common/baseDualSimplex.tpp:443-501
  *   ds_pivot_row      pivotRow -> Matrix::dotUpdate(UPDATE_T)
  *                                                   cpu/sequentialDualSimplex.cpp:95-104,
  *                                                   cpu/matrix.cpp:273-287; launcher spmvUpdateInc,
@@ -69,7 +70,24 @@ typedef struct {
   int verbose;
   int mark_iteration;     /* > 0: synchronise after this iteration and record the time (ds_result.mark_ms) */
   int time_kernels;       /* k > 0: time the pivot-row SpMV of every k-th iteration after the mark with HIP events */
+  int cleanup;            /* 1 (default): after a cost perturbation, if the restored costs leave dual
+                             infeasibilities, resume the dual simplex from the final basis (phase 1, then
+                             phase 2).  0: the reference's behaviour (its primal correction is a stub,
+                             sequentialDualSimplex.cpp:113-116, so that basis is returned) */
+  int profile;            /* 1: per-step device times in ds_result.step_ms (HIP events between the steps
+                             of every iteration, one iteration per host synchronisation) */
 } ds_params;
+
+/* Steps of ds_result.step_ms, the parts of the reference's TimeProfiler
+ * (common/types.hpp:12-27, common/timeProfiler.cpp:22-148).  The GPU fuses
+ * the candidate set into the choice (both under RATIO_TEST2) and the
+ * reduced-cost update with the basis change (under UPDATE_RED_COSTS; a basis
+ * change finished after a stability refactor is under BASIS_UPDATE). */
+enum {
+  DS_PART_INIT = 0, DS_PART_FACTOR, DS_PART_RESTORE, DS_PART_BTRAN, DS_PART_PIVOT_ROW, DS_PART_RATIO_TEST1,
+  DS_PART_RATIO_TEST2, DS_PART_FTRAN, DS_PART_FTRAN_BFRT, DS_PART_FTRAN_BETA, DS_PART_PRICING,
+  DS_PART_UPDATE_RED_COSTS, DS_PART_BASIS_UPDATE, DS_NPARTS
+};
 
 typedef struct {
   int status;
@@ -88,6 +106,9 @@ typedef struct {
   int kernel_k;           /* basis kernel size at the last refactor */
   double nnz_nonbasic_mark; /* nnz of the nonbasic columns at the mark and at the end */
   double nnz_nonbasic_end;
+  int cleanups;           /* dual simplex resumptions after restoring perturbed costs */
+  int pad0;
+  double step_ms[DS_NPARTS]; /* with params.profile: time per step (DS_PART_*) */
 } ds_result;
 
 void ds_default_params(ds_params* p);
@@ -123,6 +144,19 @@ int ds_ftran(ds_ctx* ctx, const double* b /* m, by row */, double* x /* m, by po
 int ds_ftran_column(ds_ctx* ctx, int j, double* x /* m */);
 int ds_pivot_row(ds_ctx* ctx, const double* rho /* m */, double* alpha /* n, by nonbasic position */);
 int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis /* n */);
+
+/* CHUZR on caller-owned DEVICE arrays of the m basic variables, by basis
+ * position: bound type (DS_FIXED..DS_LOWER), value, bounds and DSE weight.
+ * Replaces BaseDualSimplex::DSEPricing + checkVarIsInfeas
+ * (common/baseDualSimplex.tpp:443-501): the position maximising delta^2/beta
+ * among variables out of bounds (fallthrough rules of checkVarIsInfeas, |delta|
+ * above 1e-8), ties to the lowest position.  Writes the position (-1 when no
+ * variable is out of bounds) to the device int *p_idx and the signed
+ * infeasibility delta (x - bound; delta < 0 is the reference's is_lower) to the
+ * device double *delta.  Runs on the context's stream and returns when done;
+ * its scratch (per-workgroup partials) is allocated on the first call. */
+int ds_chuzr(ds_ctx* ctx, int m, const signed char* bt_B, const double* x_B, const double* l_B, const double* u_B,
+             const double* beta, int* p_idx, double* delta);
 
 /* Timing of the pivot-row SpMV on the context's current basis and a given rho:
  * `reps` launches, HIP events on the context stream; returns the mean ms per

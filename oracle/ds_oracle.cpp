@@ -258,7 +258,27 @@ struct Eta {
   int p;
 };
 
-/* linalg::PFIsolve (linalg.cpp:12-46). */
+/* checkVarIsInfeas with its fallthrough (tpp:443-466). */
+bool var_infeas_rule(int t, double xv, double lo, double up, double& delta) {
+  switch (t) {
+    case ORA_LOWER:
+      if (xv < lo - EPS_BOUND) { delta = xv - lo; return true; }
+      /* fallthrough */
+    case ORA_UPPER:
+      if (xv > up + EPS_BOUND) { delta = xv - up; return true; }
+      /* fallthrough */
+    case ORA_FIXED:
+      if (std::fabs(xv - lo) < EPS_BOUND) { delta = xv - lo; return true; }
+      /* fallthrough */
+    case ORA_BOXED:
+      if (xv > up + EPS_BOUND) { delta = xv - up; return true; }
+      else if (xv < lo - EPS_BOUND) { delta = xv - lo; return true; }
+  }
+  delta = 0.0;
+  return false;
+}
+
+
 /* Harris two-pass selection over candidate positions F (ascending): pass 1
  * bounds the step by the EPS_D-relaxed ratios, pass 2 takes the largest
  * |alpha| among the candidates whose plain ratio is within the bound; the
@@ -285,6 +305,7 @@ int harris_pick(int nF, const int* F, DOf dof, const double* alpha) {
   return q_idx;
 }
 
+/* linalg::PFIsolve (linalg.cpp:12-46). */
 void pfi_apply(const std::vector<Eta>& etas, std::vector<double>& sol, bool transpose) {
   int m = (int)sol.size();
   if (!transpose) {
@@ -503,25 +524,7 @@ struct Solver {
   }
 
   /* checkVarIsInfeas with its fallthrough (tpp:443-466). */
-  bool varInfeas(int j, double& delta) const {
-    double xv = x[j], lo = M.l[j], up = M.u[j];
-    switch (M.bt[j]) {
-      case ORA_LOWER:
-        if (xv < lo - EPS_BOUND) { delta = xv - lo; return true; }
-        /* fallthrough */
-      case ORA_UPPER:
-        if (xv > up + EPS_BOUND) { delta = xv - up; return true; }
-        /* fallthrough */
-      case ORA_FIXED:
-        if (std::fabs(xv - lo) < EPS_BOUND) { delta = xv - lo; return true; }
-        /* fallthrough */
-      case ORA_BOXED:
-        if (xv > up + EPS_BOUND) { delta = xv - up; return true; }
-        else if (xv < lo - EPS_BOUND) { delta = xv - lo; return true; }
-    }
-    delta = 0.0;
-    return false;
-  }
+  bool varInfeas(int j, double& delta) const { return var_infeas_rule(M.bt[j], x[j], M.l[j], M.u[j], delta); }
 
   /* DSEPricing (tpp:470-501). */
   void dsePricing(const std::vector<char>& blocked, int& p, int& p_idx, double& delta, bool& is_lower) {
@@ -933,10 +936,15 @@ struct Solver {
     if (perturbed) {
       c = original_costs;
       dualSimplexInit();
+      /* dualSimplexInit places boxed nonbasic variables by the sign of the reduced costs it
+       * finds on entry (setPrimalVars runs before initReducedCosts, seq.cpp:230-250), here
+       * those of the perturbed costs; the cleanup places them again by the restored ones. */
+      if (P.cleanup) dualSimplexInit();
       /* postProcess (tpp:123-159) calls callPrimalSolver when the restored
        * costs leave dual infeasibilities; that correction is a stub in the
        * reference (sequentialDualSimplex.cpp:113-116).  cleanup = 1 resumes
-       * the dual simplex from the final basis instead. */
+       * the dual simplex from the final basis instead: phase 1 + phase 2 when the
+       * original costs leave dual infeasibilities, phase 2 when only x_B is out of bounds. */
       if (P.cleanup && ok && !checkDualFeasible()) {
         int st2 = phase1();
         if (st2 == 1) {
@@ -946,6 +954,10 @@ struct Solver {
           return;
         }
         setPrimalVars();
+        ok = phase2();
+      } else if (P.cleanup && ok && !checkPrimalFeasible()) {
+        /* Dual feasible for the original costs, but dualSimplexInit moved boxed nonbasic
+         * variables whose reduced cost changed sign to their other bound: continue phase 2. */
         ok = phase2();
       }
     }
@@ -1117,4 +1129,24 @@ extern "C" int ora_simple_ratio_test(int nF, const int* F, const double* d_of_po
 
 extern "C" int ora_harris_ratio_test(int nF, const int* F, const double* d_of_pos, const double* alpha) {
   return harris_pick(nF, F, [&](int i) { return d_of_pos[i]; }, alpha);
+}
+
+extern "C" int ora_dse_pricing(int m, const signed char* bt, const double* x, const double* l, const double* u,
+                               const double* beta, int* p_idx, double* delta) {
+  /* DSEPricing (tpp:470-501) over arrays by basis position. */
+  double best = 0.0;
+  *p_idx = -1;
+  *delta = 0.0;
+  for (int i = 0; i < m; ++i) {
+    double dt;
+    if (var_infeas_rule(bt[i], x[i], l[i], u[i], dt) && std::fabs(dt) > DSE_FLOOR) {
+      double w = (dt * dt) / beta[i];
+      if (w > best) {
+        *p_idx = i;
+        *delta = dt;
+        best = w;
+      }
+    }
+  }
+  return 0;
 }

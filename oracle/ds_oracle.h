@@ -107,6 +107,12 @@ int ora_simple_ratio_test(int nF, const int* F, const double* d_of_pos, const do
  * position on ties); returns q_idx or -1. */
 int ora_harris_ratio_test(int nF, const int* F, const double* d_of_pos, const double* alpha);
 
+/* DSEPricing (baseDualSimplex.tpp:470-501) over arrays by basis position:
+ * *p_idx = argmax delta^2/beta (strict, ascending positions) among the
+ * variables checkVarIsInfeas flags with |delta| > 1e-8, or -1. */
+int ora_dse_pricing(int m, const signed char* bt, const double* x, const double* l, const double* u,
+                    const double* beta, int* p_idx, double* delta);
+
 #ifdef __cplusplus
 }
 #endif

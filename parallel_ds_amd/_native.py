@@ -18,7 +18,13 @@ ERRORS = {0: "DS_OK", -1: "DS_EINVAL", -2: "DS_EHIP", -3: "DS_ESINGULAR", -4: "D
 class Params(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
                 ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
-                ("mark_iteration", C.c_int), ("time_kernels", C.c_int)]
+                ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
+                ("profile", C.c_int)]
+
+
+# Steps of Result.step_ms (the reference's TimeProfiler parts, common/types.hpp:12-27).
+PARTS = ("Init", "Factor", "RestoreProc", "Btran", "PivotRow", "RatioTestPart1", "RatioTestPart2", "Ftran",
+         "FtranBfrt", "FtranBeta", "Pricing", "UpdateRedCosts", "BasisUpdate")
 
 
 class Result(C.Structure):
@@ -26,7 +32,8 @@ class Result(C.Structure):
                 ("phase1_iterations", C.c_int), ("perturbed", C.c_int), ("refactors", C.c_int),
                 ("restarts", C.c_int), ("Z", C.c_double), ("solve_ms", C.c_double), ("refactor_ms", C.c_double),
                 ("mark_ms", C.c_double), ("spmv_ms", C.c_double), ("spmv_launches", C.c_int), ("kernel_k", C.c_int),
-                ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double)]
+                ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double), ("cleanups", C.c_int),
+                ("pad0", C.c_int), ("step_ms", C.c_double * 13)]
 
 
 class BatchParams(C.Structure):
@@ -62,6 +69,7 @@ _SIGS = {
     "ds_get_solution": (I32, [VP, C.POINTER(F64), C.POINTER(I32), C.c_char_p, I32, VP]),
     "ds_get_basis": (I32, [VP, VP]),
     "ds_get_nonbasis": (I32, [VP, VP]),
+    "ds_chuzr": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP]),
     "ds_solve_lp": (I32, [VP, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.POINTER(Params),
                           C.POINTER(Result), VP, VP]),
     "ds_set_basis": (I32, [VP, VP]),

@@ -10,13 +10,7 @@
 #include "ds_solver.hpp"
 #include "ds_warm.hpp"
 
-struct ds_ctx {
-  pds::Solver* solver = nullptr;
-  pds::BatchSolver* batch = nullptr;
-  pds::WarmBuffers* wb = nullptr;
-  int device = 0;
-  std::string err;
-};
+#include "ds_ctx.hpp"
 
 struct ds_mlp {
   pds::Mlp* net = nullptr;
@@ -100,6 +94,8 @@ void ds_default_params(ds_params* p) {
   p->verbose = 0;
   p->mark_iteration = 0;
   p->time_kernels = 0;
+  p->cleanup = 1;
+  p->profile = 0;
 }
 
 int ds_create(int device, ds_ctx** out) {
@@ -122,6 +118,8 @@ int ds_create(int device, ds_ctx** out) {
 
 int ds_destroy(ds_ctx* ctx) {
   if (!ctx) return DS_EINVAL;
+  if (ctx->chuzr_part_w) (void)hipFree(ctx->chuzr_part_w);
+  if (ctx->chuzr_part_i) (void)hipFree(ctx->chuzr_part_i);
   delete ctx->wb;
   delete ctx->batch;
   delete ctx->solver;

@@ -36,7 +36,7 @@ enum : int {
   ST_PH1_DUALINF = 7,
   ST_PH1_RESTART = 8,      // phase 1: no candidate row or Z > 0
   ST_PH1_REINIT = 9,       // phase 1: cycle > NEED_RESTART or Z blew up
-  ST_BFRT = 10,            // boxed entering variable needs a bound flip (host path)
+  ST_BFRT = 10,            // (unused: bound flips run on the device)
   ST_BTRAN_CAP = 11,       // sparse BTRAN list overflowed its LDS capacity
   ST_ERROR = 12
 };
@@ -58,15 +58,18 @@ struct DevScalars {
   int any_blocked;
   int nblocked;
   int k;             // kernel size of the current factor
+  int bfrt;          // the entering candidate is boxed and flips: the bound-flipping loop runs
+  int nflip;         // boxed nonbasic variables that flip in this iteration
   int pad0;
   double delta, theta, obj, f_p, corr, alpha_p, alpha_rq, theta_P;
   double tmp0, tmp1;
+  double dz;         // objective change of the flips' costs (first part of delta_z)
   int ctr[16];       // last-workgroup counters, reset by the finaliser
 };
 
 constexpr int MAX_BLOCKED = 64;
 
 // Counter slots in DevScalars::ctr.
-enum { CTR_PRICE = 0, CTR_CHUZC1, CTR_CHUZC2, CTR_DUPD, CTR_UPD, CTR_DCHK, CTR_DOT, CTR_INF, CTR_SPMV };
+enum { CTR_PRICE = 0, CTR_CHUZC1, CTR_CHUZC2, CTR_DUPD, CTR_UPD, CTR_DCHK, CTR_DOT, CTR_INF, CTR_SPMV, CTR_FLIP, CTR_BXB };
 
 }  // namespace pds

@@ -44,6 +44,13 @@ struct DevIter {
   int* cand_i;       // textbook CHUZC candidate lists, TB_LIST per workgroup
   double* cand_th;
   double* cand_a;
+  // Bound-flipping ratio test (models with boxed variables only):
+  unsigned char* bmark;    // n: positions dropped by the flip loop (cleared after the loop)
+  int* blist;              // n: those positions, in drop order
+  signed char* flipc;      // n: +1 flips lower -> upper, -1 upper -> lower, 0 none
+  int* flist;              // n: flipping positions in ascending order
+  double* bcc;             // m: column change sum_j (+-)(u_j - l_j) a_j of the flips
+  double* dxb;             // m: B^{-1} bcc
   int eta_cap;
   unsigned char* blocked;  // N
   int* blocked_list;       // MAX_BLOCKED

@@ -17,6 +17,12 @@ constexpr int BS = 256;
 
 __device__ inline bool stopped(const DevScalars* S) { return S->status != ST_RUNNING || S->refac_mid != 0; }
 
+// Launch guard of the queued kernels: check 0 = always run, 1 = not after an
+// event, 2 = also not when the iteration has no bound flips (the BFRT FTRAN).
+__device__ inline bool skip_launch(const DevScalars* S, int check) {
+  return check && (stopped(S) || (check == 2 && S->nflip == 0));
+}
+
 __device__ inline void raise_status(DevScalars* S, int st, int it) {
   S->status = st;
   S->stop_iter = it;
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
 
 __global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFactor F, double* out, int check) {
   DevScalars* S = I.S;
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   __shared__ int sidx[1024];
   __shared__ double sval[1024];
   __shared__ int yrow[ETA_MAX + 1];
@@ -454,9 +460,11 @@ __device__ inline void chuzc_finish(const DevModel& M, const DevIter& I, int pha
   S->theta = I.d[q] / arq;
   if (phase == 2) {
     S->unbound_checked = 0;
+    S->bfrt = 0;
     if (M.bt[q] == BT_BOXED) {
+      // The bound-flipping loop (k_bfrt) continues from this candidate.
       double span = M.u[q] - M.l[q];
-      if (fabs(S->delta) - span * fabs(arq) > 0) raise_status(S, ST_BFRT, it);
+      if (fabs(S->delta) - span * fabs(arq) > 0) S->bfrt = 1;
     }
   } else {
     int8_t tq = M.bt[q];
@@ -652,7 +660,7 @@ constexpr int FT_MAXCOL = 512;
 __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFactor F, int col, double* out,
                                                   int check) {
   DevScalars* S = I.S;
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   const int j = col >= 0 ? col : S->q;
   __shared__ int er[FT_MAXCOL];
   __shared__ int eri[FT_MAXCOL];
@@ -722,7 +730,7 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
 __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* __restrict__ x0,
                                                   double* out, int t, int mode, int check) {
   DevScalars* S = I.S;
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   __shared__ double xm[ETA_MAX];
   __shared__ double sk[ETA_MAX];
   __shared__ int pk[ETA_MAX];
@@ -755,7 +763,7 @@ __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const d
 // Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
 __global__ __launch_bounds__(BS) void k_gather(const double* src, const int* map, double* dst, int len,
                                                const DevScalars* S, int check) {
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   int i = blockIdx.x * BS + threadIdx.x;
   if (i < len) dst[i] = src[map[i]];
 }
@@ -764,7 +772,7 @@ __global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int r
                                              const double* __restrict__ xin, const int* imap,
                                              double* __restrict__ dst, const int* omap, const DevScalars* S,
                                              int check) {
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   const int w = (blockIdx.x * BS + threadIdx.x) / WAVE;
   const int lane = threadIdx.x % WAVE;
   if (w >= rows) return;
@@ -780,7 +788,7 @@ __global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int r
 
 __global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, const double* b, double* dst,
                                                     const DevScalars* S, int check) {
-  if (check && stopped(S)) return;
+  if (skip_launch(S, check)) return;
   int r = blockIdx.x * BS + threadIdx.x;
   if (r >= M.m) return;
   int sp = F.slackpos_of_row[r];
@@ -797,24 +805,20 @@ __global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, con
 // Grid-stride part of the dual update: d_j -= theta * alpha_r over the
 // nonbasic set, with the dual-infeasibility and bound-flip counts.
 __device__ __forceinline__ void dupdate_body(const DevModel& M, const DevIter& I, int phase, double theta, int q_idx,
-                                             int& dinf, int& flips) {
+                                             int& dinf) {
   const int nN = M.N - M.m;
   for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
     int j = I.nonbasis[i];
     double dj = I.d[j] - theta * I.alpha_r[i];
     I.d[j] = dj;
     if (i != q_idx) dinf += dual_infeas(M.bt[j], dj);
-    if (phase == 2 && M.bt[j] == BT_BOXED) {
-      double xj = I.x[j];
-      if ((fabs(xj - M.l[j]) < EPS_Z && dj < -EPS_D) || (fabs(xj - M.u[j]) < EPS_Z && dj > EPS_D)) ++flips;
-    }
   }
 }
 
 // Thread 0 of the last workgroup: entering/leaving duals, the stability check
 // of alpha_q[p] against alpha_r[q], bound flips, objective.
 __device__ __forceinline__ void dupdate_final(const DevModel& M, const DevIter& I, int phase, int it, double theta,
-                                              int td, int tf) {
+                                              int td) {
   DevScalars* S = I.S;
   const int p = S->p, q = S->q;
   I.d[p] = -theta;
@@ -826,7 +830,6 @@ __device__ __forceinline__ void dupdate_final(const DevModel& M, const DevIter& 
     S->stop_iter = it;
   }
   if (phase == 2) {
-    if (tf > 0) raise_status(S, ST_BFRT, it);
     S->obj = S->obj + theta * S->delta;
   } else {
     S->obj = S->obj - theta * S->f_p;
@@ -838,25 +841,17 @@ __global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase
   DevScalars* S = I.S;
   if (stopped(S)) return;
   const double theta = S->theta;
-  int dinf = 0, flips = 0;
-  dupdate_body(M, I, phase, theta, S->q_idx, dinf, flips);
+  int dinf = 0;
+  dupdate_body(M, I, phase, theta, S->q_idx, dinf);
   dinf = block_sumi<BS>(dinf);
-  flips = block_sumi<BS>(flips);
-  if (threadIdx.x == 0) {
-    I.part_i[2 * blockIdx.x] = dinf;
-    I.part_i[2 * blockIdx.x + 1] = flips;
-  }
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = dinf;
   if (!last_block(&S->ctr[CTR_DUPD])) return;
-  int td = 0, tf = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
-    td += I.part_i[2 * b];
-    tf += I.part_i[2 * b + 1];
-  }
+  int td = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) td += I.part_i[b];
   td = block_sumi<BS>(td);
-  tf = block_sumi<BS>(tf);
   if (threadIdx.x != 0) return;
   S->ctr[CTR_DUPD] = 0;
-  dupdate_final(M, I, phase, it, theta, td, tf);
+  dupdate_final(M, I, phase, it, theta, td);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,28 +977,233 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
   const double theta = S->theta;
   const UpdateArgs u = update_args(M, I, phase);
   const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
-  int dinf = 0, flips = 0;
-  dupdate_body(M, I, phase, theta, S->q_idx, dinf, flips);
+  int dinf = 0;
+  dupdate_body(M, I, phase, theta, S->q_idx, dinf);
   if (stable) update_body(M, I, phase, u);
   dinf = block_sumi<BS>(dinf);
-  flips = block_sumi<BS>(flips);
-  if (threadIdx.x == 0) {
-    I.part_i[2 * blockIdx.x] = dinf;
-    I.part_i[2 * blockIdx.x + 1] = flips;
-  }
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = dinf;
   if (!last_block(&S->ctr[CTR_DUPD])) return;
-  int td = 0, tf = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
-    td += I.part_i[2 * b];
-    tf += I.part_i[2 * b + 1];
-  }
+  int td = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) td += I.part_i[b];
   td = block_sumi<BS>(td);
-  tf = block_sumi<BS>(tf);
   if (threadIdx.x == 0) {
     S->ctr[CTR_DUPD] = 0;
-    dupdate_final(M, I, phase, it, theta, td, tf);
+    dupdate_final(M, I, phase, it, theta, td);
   }
   if (stable) update_final(M, I, phase, u);
+}
+
+// ---------------------------------------------------------------------------
+// Bound-flipping ratio test and BFRT FTRAN (sequentialDualSimplex.cpp:398-482),
+// queued only for models with boxed variables.
+// ---------------------------------------------------------------------------
+constexpr int BFRT_BS = 1024;
+
+// Exclusive prefix count of `flag` over the 1024 threads of a workgroup, in
+// thread order; *total receives the count of the whole workgroup.
+__device__ inline int block_excl_count(int flag, int* total) {
+  __shared__ int wsum[BFRT_BS / WAVE];
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  const unsigned long long b = __ballot(flag);
+  const int pre = __popcll(b & ((1ull << lane) - 1ull));
+  __syncthreads();
+  if (lane == 0) wsum[w] = __popcll(b);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int k = 0; k < BFRT_BS / WAVE; ++k) {
+    off += (k < w) ? wsum[k] : 0;
+    tot += wsum[k];
+  }
+  *total = tot;
+  return off + pre;
+}
+
+// The flip loop, one workgroup.  CHUZC chose a boxed candidate q whose flip
+// alone does not absorb the leaving row's infeasibility delta: drop it
+// (delta -= (u_q - l_q)|alpha_q|) and choose again among the remaining
+// candidates with the same rule (textbook or Harris), until the choice is not
+// boxed, or its flip would absorb what is left, or no candidate remains (then
+// the last dropped one enters, as in the reference).  Writes q, theta and the
+// signed remaining delta.
+__global__ __launch_bounds__(BFRT_BS) void k_bfrt(DevModel M, DevIter I, int harris) {
+  DevScalars* S = I.S;
+  if (stopped(S) || !S->bfrt) return;
+  const int neg = S->neg;
+  const int nN = M.N - M.m;
+  __shared__ double s_delta;
+  __shared__ int s_nrm, s_qi, s_go;
+  if (threadIdx.x == 0) {
+    const int q0 = S->q_idx, j0 = S->q;
+    s_delta = fabs(S->delta) - (M.u[j0] - M.l[j0]) * fabs(I.alpha_r[q0]);
+    I.bmark[q0] = 1;
+    I.blist[0] = q0;
+    s_nrm = 1;
+    s_qi = q0;
+    s_go = 1;
+  }
+  __syncthreads();
+  while (s_go) {
+    // pass 1: bound of the step over the remaining candidates
+    double mn = BIG;
+    for (int i = threadIdx.x; i < nN; i += BFRT_BS) {
+      if (I.bmark[i]) continue;
+      double a = I.alpha_r[i];
+      if (neg) a = -a;
+      const int j = I.nonbasis[i];
+      if (!ratio_candidate(M, I, 2, j, a)) continue;
+      const double dj = I.d[j];
+      mn = fmin(mn, harris ? (a > 0 ? (dj + EPS_D) / a : (dj - EPS_D) / a) : dj / a);
+    }
+    mn = block_min<BFRT_BS>(mn);
+    // pass 2: largest |alpha| within the bound, lowest position on ties
+    double best = -1.0;
+    int bi = INT_MAX;
+    for (int i = threadIdx.x; i < nN; i += BFRT_BS) {
+      if (I.bmark[i]) continue;
+      double a = I.alpha_r[i];
+      if (neg) a = -a;
+      const int j = I.nonbasis[i];
+      if (!ratio_candidate(M, I, 2, j, a)) continue;
+      const double th = I.d[j] / a;
+      if (harris ? (th <= mn) : (fabs(th - mn) < EPS_Z)) argmax_merge(best, bi, fabs(a), i);
+    }
+    block_argmax<BFRT_BS>(best, bi);
+    if (threadIdx.x == 0) {
+      if (bi == INT_MAX) {
+        s_go = 0;  // every candidate dropped: the last one enters
+      } else {
+        const int j = I.nonbasis[bi];
+        s_qi = bi;
+        const double span = M.u[j] - M.l[j];
+        if (M.bt[j] == BT_BOXED && s_delta - span * fabs(I.alpha_r[bi]) > 0) {
+          s_delta -= span * fabs(I.alpha_r[bi]);
+          I.bmark[bi] = 1;
+          I.blist[s_nrm++] = bi;
+        } else {
+          s_go = 0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < s_nrm; k += BFRT_BS) I.bmark[I.blist[k]] = 0;
+  if (threadIdx.x != 0) return;
+  const int qi = s_qi, q = I.nonbasis[qi];
+  const double arq = I.alpha_r[qi];
+  S->q_idx = qi;
+  S->q = q;
+  S->alpha_rq = arq;
+  S->theta = I.d[q] / arq;
+  S->delta = (S->delta < 0) ? -s_delta : s_delta;
+  S->bfrt = 0;
+}
+
+// Boxed nonbasic variables whose reduced cost changes sign in the dual update
+// (the same d_j - theta alpha_j the update computes): +1 at the lower bound
+// with d_j < -EPS_D (flips to the upper), -1 at the upper with d_j > EPS_D.
+__global__ __launch_bounds__(BS) void k_flip_detect(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double theta = S->theta;
+  const int nN = M.N - M.m;
+  int cnt = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    const int j = I.nonbasis[i];
+    signed char code = 0;
+    if (M.bt[j] == BT_BOXED) {
+      const double dj = I.d[j] - theta * I.alpha_r[i];
+      const double xj = I.x[j];
+      if (fabs(xj - M.l[j]) < EPS_Z && dj < -EPS_D) code = 1;
+      else if (fabs(xj - M.u[j]) < EPS_Z && dj > EPS_D) code = -1;
+    }
+    I.flipc[i] = code;
+    cnt += code != 0;
+  }
+  cnt = block_sumi<BS>(cnt);
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = cnt;
+  if (!last_block(&S->ctr[CTR_FLIP])) return;
+  int tot = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) tot += I.part_i[b];
+  tot = block_sumi<BS>(tot);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_FLIP] = 0;
+    S->nflip = tot;
+  }
+}
+
+// One workgroup: the flips in ascending position (the reference's loop
+// order), their cost change dz = sum c_j (u_j - l_j) (+ for lower -> upper,
+// - for upper -> lower), and the column change
+//   bcc = sum_{lower->upper} (u_j - l_j) a_j - sum_{upper->lower} (u_j - l_j) a_j
+// accumulated column after column (Matrix::addSparseCol, matrix.cpp:606-616).
+__global__ __launch_bounds__(BFRT_BS) void k_flip_cc(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  if (skip_launch(S, 2)) return;
+  const int nN = M.N - M.m;
+  for (int r = threadIdx.x; r < M.m; r += BFRT_BS) I.bcc[r] = 0.0;
+  __shared__ int s_base;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (int i0 = 0; i0 < nN; i0 += BFRT_BS) {
+    const int i = i0 + threadIdx.x;
+    const int f = (i < nN && I.flipc[i] != 0) ? 1 : 0;
+    int tot;
+    const int off = block_excl_count(f, &tot);
+    if (f) I.flist[s_base + off] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) s_base += tot;
+    __syncthreads();
+  }
+  // The sums run in one thread, in the reference's order (a column's entries
+  // may repeat a row); a flip touches a handful of entries.
+  if (threadIdx.x != 0) return;
+  const int nf = s_base;
+  double dz = 0.0;
+  for (int k = 0; k < nf; ++k) {
+    const int i = I.flist[k], j = I.nonbasis[i];
+    const double span = M.u[j] - M.l[j];
+    if (I.flipc[i] > 0) dz += M.c[j] * span;
+    else dz -= M.c[j] * span;
+  }
+  S->dz = dz;
+  for (int sgn = 1; sgn >= -1; sgn -= 2) {
+    const double mult = (double)sgn;
+    for (int k = 0; k < nf; ++k) {
+      const int i = I.flist[k];
+      if (I.flipc[i] != sgn) continue;
+      const int j = I.nonbasis[i];
+      const double span = M.u[j] - M.l[j];
+      for (int e = M.cptr[j]; e < M.cptr[j + 1]; ++e) I.bcc[M.ridx[e]] += M.cval[e] * span * mult;
+    }
+  }
+}
+
+// x_B -= B^{-1} bcc, delta_z = dz - c_B . dx_B added to the objective, and the
+// flipped variables moved to their other bound.
+__global__ __launch_bounds__(BS) void k_bfrt_xb(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  if (skip_launch(S, 2)) return;
+  double s = 0.0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    const int j = I.basis[i];
+    const double v = I.dxb[i];
+    I.x[j] = I.x[j] - v;
+    s += M.c[j] * v;
+  }
+  const int nf = S->nflip;
+  for (int k = blockIdx.x * BS + threadIdx.x; k < nf; k += gridDim.x * BS) {
+    const int i = I.flist[k], j = I.nonbasis[i];
+    I.x[j] = I.flipc[i] > 0 ? M.u[j] : M.l[j];
+  }
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) I.part_d[blockIdx.x] = s;
+  if (!last_block(&S->ctr[CTR_BXB])) return;
+  if (threadIdx.x == 0) {
+    double g = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b) g += I.part_d[b];
+    S->ctr[CTR_BXB] = 0;
+    S->obj = S->obj + (S->dz - g);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1273,6 +1473,16 @@ void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   k_dupdate_update<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
+}
+void launch_bfrt(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int harris, int t) {
+  k_bfrt<<<1, BFRT_BS, 0, s>>>(M, I, harris);
+}
+void launch_flips(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t) {
+  k_flip_detect<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I);
+  k_flip_cc<<<1, BFRT_BS, 0, s>>>(M, I);
+  launch_ftran_dense(s, M, I, F, I.bcc, I.dxb, t, 0, 2);
+  const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
+  k_bfrt_xb<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I);
 }
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I) {
   k_set_primal<<<nblk(M.N - M.m, BS), BS, 0, s>>>(M, I);

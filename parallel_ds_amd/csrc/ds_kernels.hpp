@@ -23,6 +23,10 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
 void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
+// Bound-flipping ratio test: the flip loop after CHUZC, then the flip set, the
+// column change, its FTRAN and the x_B / objective correction.
+void launch_bfrt(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int harris, int t);
+void launch_flips(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t);
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I);
 void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN);
 void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,

@@ -116,6 +116,43 @@ void Solver::set_params(const ds_params& p) {
     }
     ev_iter_.assign(P_.chunk, 0);
   }
+  if (P_.profile && pev_.empty()) {
+    for (int i = 0; i < 16; ++i) {
+      hipEvent_t e;
+      PDS_HIP(hipEventCreate(&e));
+      pev_.push_back(e);
+    }
+  }
+}
+
+// Per-step profile: events recorded between the steps of one queued iteration;
+// after the host synchronises, the interval up to each event goes to its step.
+void Solver::prof_event(int part) {
+  if (!P_.profile) return;
+  const int k = (int)pev_part_.size();
+  if (k >= (int)pev_.size()) return;
+  PDS_HIP(hipEventRecord(pev_[k], st_));
+  pev_part_.push_back(part);
+}
+void Solver::prof_harvest() {
+  if (!P_.profile || pev_part_.empty()) return;
+  PDS_HIP(hipEventSynchronize(pev_[pev_part_.size() - 1]));
+  for (size_t k = 1; k < pev_part_.size(); ++k) {
+    float ms = 0.f;
+    PDS_HIP(hipEventElapsedTime(&ms, pev_[k - 1], pev_[k]));
+    step_ms_[pev_part_[k]] += ms;
+  }
+  pev_part_.clear();
+}
+void Solver::prof_host_begin() {
+  if (!P_.profile) return;
+  sync();
+  prof_t0_ = now_ms();
+}
+void Solver::prof_host_end(int part) {
+  if (!P_.profile) return;
+  sync();
+  step_ms_[part] += now_ms() - prof_t0_;
 }
 
 // Sum the SpMV event times of the queued iterations that ran (<= last_done).
@@ -190,6 +227,12 @@ DevIter Solver::I() const {
   it.cand_i = cand_i_.p;
   it.cand_th = cand_th_.p;
   it.cand_a = cand_a_.p;
+  it.bmark = bmark_.p;
+  it.blist = blist_.p;
+  it.flipc = flipc_.p;
+  it.flist = flist_.p;
+  it.bcc = bcc_.p;
+  it.dxb = dxb_.p;
   it.eta_cap = eta_cap_;
   it.blocked = blocked_.p;
   it.blocked_list = blocked_list_.p;
@@ -322,6 +365,16 @@ void Solver::load(const HostModel& hm) {
   rowsB_.alloc((size_t)(m / 256 + 2) * GJ_NB);
   nA_.alloc(m / 256 + 2);
   nB_.alloc(m / 256 + 2);
+  has_boxed_ = false;
+  for (int j = 0; j < N; ++j) has_boxed_ = has_boxed_ || H_.bt[j] == BT_BOXED;
+  const size_t nb = has_boxed_ ? (size_t)n : 1, mb = has_boxed_ ? (size_t)m : 1;
+  bmark_.alloc(nb);
+  blist_.alloc(nb);
+  flipc_.alloc(nb);
+  flist_.alloc(nb);
+  bcc_.alloc(mb);
+  dxb_.alloc(mb);
+  PDS_HIP(hipMemsetAsync(bmark_.p, 0, nb, st_));
   PDS_HIP(hipMemsetAsync(blocked_.p, 0, N, st_));
   PDS_HIP(hipMemsetAsync(E_.p, 0, sizeof(double) * eta_cap_ * eta_cap_, st_));
   sync();
@@ -548,7 +601,9 @@ void Solver::initDualSimplex() {
   rng_state_ = 0;
   iteration_ = 0;
   t_ = 0;
-  refactors_ = restarts_ = phase1_iters_ = 0;
+  refactors_ = restarts_ = phase1_iters_ = cleanups_ = 0;
+  for (double& v : step_ms_) v = 0.0;
+  pev_part_.clear();
   solve_ms_ = refactor_ms_ = 0.0;
   mark_ms_ = spmv_ms_ = 0.0;
   spmv_launches_ = 0;
@@ -566,9 +621,12 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   const DevModel m = M();
   const DevIter i = I();
   const DevFactor f = F();
+  prof_event(DS_PART_PRICING);
   if (phase == 2 && dualcheck) launch_dualcheck(st_, m, i, it);
   launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
+  prof_event(DS_PART_PRICING);
   launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  prof_event(DS_PART_BTRAN);
   // Timed launches carry a start/stop event pair that records the kernel's own
   // execution interval (hipExtLaunchKernelGGL).
   // Every time_kernels-th iteration: a per-launch event pair costs the loop a few percent.
@@ -576,16 +634,31 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1, timed ? ev_[2 * ev_used_] : nullptr,
               timed ? ev_[2 * ev_used_ + 1] : nullptr);
   if (timed) ev_iter_[ev_used_++] = it;
+  prof_event(DS_PART_PIVOT_ROW);
   launch_chuzc(st_, m, i, phase, P_.ratio_test, it);
+  // Bound flips (sequentialDualSimplex.cpp:398-422) exist only with boxed variables.
+  const bool flips = phase == 2 && has_boxed_;
+  if (flips) launch_bfrt(st_, m, i, f, P_.ratio_test, t);
+  prof_event(DS_PART_RATIO_TEST2);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
+  prof_event(DS_PART_FTRAN);
   // tau = B^-1 rho first: the dual update and the basis change then go in one launch.
   launch_ftran_dense(st_, m, i, f, rho_.p, tau_.p, t, 0, 1);
+  prof_event(DS_PART_FTRAN_BETA);
+  if (flips) {
+    launch_flips(st_, m, i, f, t);
+    prof_event(DS_PART_FTRAN_BFRT);
+  }
   launch_dupdate_update(st_, m, i, phase, it);
+  prof_event(DS_PART_UPDATE_RED_COSTS);
 }
 
 void Solver::enqueue_stageB(int phase, int it, int t) {
+  prof_event(DS_PART_BASIS_UPDATE);
   launch_ftran_dense(st_, M(), I(), F(), rho_.p, tau_.p, t, 0, 1);
   launch_update(st_, M(), I(), phase, it);
+  prof_event(DS_PART_BASIS_UPDATE);
+  prof_harvest();
 }
 
 // Queue iterations next, next+1, ... up to the chunk size, stopping before
@@ -603,6 +676,7 @@ int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
     bool dc = (phase == 2) && (i % NEED_RESTART == 0) && (L > 0 || skip_first == 0);
     enqueue_body(phase, i, L == 0 ? skip_first : 0, dc, t + L);
     ++L;
+    if (P_.profile) break;  // one iteration per synchronisation: the step events are read back each time
   }
   return L;
 }
@@ -642,6 +716,7 @@ int Solver::phase1() {
     }
     int L = plan_chunk(1, next, t_, skip_first);
     pull();
+    prof_harvest();
     if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
       iteration_ = next + L - 1;
       t_ = hS_->t;
@@ -666,6 +741,7 @@ int Solver::phase1() {
     hS_->status = ST_RUNNING;
     hS_->refac_mid = 0;
     push();
+    prof_host_begin();
     switch (st) {
       case ST_PH1_SOLVED:
         return 0;
@@ -724,13 +800,16 @@ int Solver::phase1() {
       default:
         throw std::runtime_error("unexpected device status " + std::to_string(st) + " in phase 1");
     }
+    prof_host_end(DS_PART_RESTORE);
   }
 }
 
 // callDualSolver (sequentialDualSimplex.cpp:253-515).
 bool Solver::phase2() {
   const int K = P_.refactor_freq;
+  prof_host_begin();
   dualSimplexInit();
+  prof_host_end(DS_PART_INIT);
   pull();
   hS_->cycle = 0;
   hS_->unbound_checked = 0;
@@ -750,6 +829,7 @@ bool Solver::phase2() {
     }
     int L = plan_chunk(2, next, t_, skip_first);
     pull();
+    prof_harvest();
     if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
       iteration_ = next + L - 1;
       t_ = hS_->t;
@@ -761,7 +841,6 @@ bool Solver::phase2() {
     harvest_events(stop - 1);
     iteration_ = stop;
     t_ = hS_->t;
-    if (hS_->status == ST_BFRT) throw Unsupported("bound flipping of boxed variables is not supported on the GPU path");
     if (hS_->refac_mid && hS_->status == ST_RUNNING) {
       hS_->refac_mid = 0;
       push();
@@ -783,17 +862,23 @@ bool Solver::phase2() {
         return true;
       case ST_NO_PIVOT:
         restarts_++;
+        prof_host_begin();
         refactor();
         initBetaWeights(false);
         dualSimplexInit();
+        prof_host_end(DS_PART_RESTORE);
         break;
       case ST_DUAL_RESTART: {
         restarts_++;
+        prof_host_begin();
         refactor();
+        prof_host_end(DS_PART_RESTORE);
         pull();
         int cyc = hS_->cycle;
         phase1();
+        prof_host_begin();
         dualSimplexInit();
+        prof_host_end(DS_PART_RESTORE);
         pull();
         hS_->cycle = cyc;
         push();
@@ -802,9 +887,11 @@ bool Solver::phase2() {
         break;
       }
       case ST_NEED_PERTURB:
+        prof_host_begin();
         perturbCosts();
         refactor();
         dualSimplexInit();
+        prof_host_end(DS_PART_RESTORE);
         resume = stop;
         resume_skip = 1;
         break;
@@ -819,12 +906,16 @@ bool Solver::phase2() {
         restarts_++;
         hS_->unbound_checked = 1;
         push();
+        prof_host_begin();
         refactor();
         initBetaWeights(false);
+        prof_host_end(DS_PART_RESTORE);
         pull();
         int cyc = hS_->cycle;
         phase1();
+        prof_host_begin();
         dualSimplexInit();
+        prof_host_end(DS_PART_RESTORE);
         pull();
         hS_->cycle = cyc;
         hS_->unbound_checked = 1;
@@ -837,6 +928,35 @@ bool Solver::phase2() {
         throw std::runtime_error("unexpected device status " + std::to_string(st) + " in phase 2");
     }
   }
+}
+
+// checkDualFeasible (tpp:362-385) on the current reduced costs.
+bool Solver::dual_feasible() {
+  pull();
+  hS_->status = ST_RUNNING;
+  hS_->refac_mid = 0;
+  push();
+  launch_dualcheck(st_, M(), I(), iteration_);
+  pull();
+  const bool ok = hS_->dinf == 0;
+  hS_->status = ST_RUNNING;
+  push();
+  return ok;
+}
+
+// checkPrimalFeasible (tpp:296-324) on the current x_B: the phase-2 pricing
+// kernel raises ST_OPTIMAL exactly when no basic variable is out of bounds.
+bool Solver::primal_feasible() {
+  pull();
+  hS_->status = ST_RUNNING;
+  hS_->refac_mid = 0;
+  push();
+  launch_price(st_, M(), I(), 2, iteration_, 1, 0);
+  pull();
+  const bool ok = hS_->status == ST_OPTIMAL;
+  hS_->status = ST_RUNNING;
+  push();
+  return ok;
 }
 
 int Solver::presolve() {
@@ -873,12 +993,36 @@ void Solver::solve() {
   const double t0 = now_ms();
   if (status_ == DS_ITERATION_LIMIT) return;
   try {
-    phase2();
+    bool ok = phase2();
     // postProcess (tpp:123-159): restore the original costs and recompute.
     if (perturbed_) {
       c_host_ = c_orig_;
       upload_costs();
       dualSimplexInit();
+      // dualSimplexInit places boxed nonbasic variables by the sign of the reduced costs it
+      // finds on entry (setPrimalVars before initReducedCosts, sequentialDualSimplex.cpp:230-250),
+      // here those of the perturbed costs; the cleanup places them again by the restored ones.
+      if (P_.cleanup && has_boxed_) dualSimplexInit();
+      // The reference's primal correction is a stub (sequentialDualSimplex.cpp:113-116): the
+      // final basis may be dual infeasible for the original costs.  With `cleanup` the dual
+      // simplex resumes from it (phase 1, then phase 2), as the batched solver does.
+      if (P_.cleanup && ok && !dual_feasible()) {
+        cleanups_++;
+        const int st1 = phase1();
+        if (st1 == 1) {
+          status_ = DS_DUAL_INFEASIBLE;
+          solved_ = false;
+          message_ = "dual infeasible";
+        } else {
+          launch_set_primal(st_, M(), I());
+          phase2();
+        }
+      } else if (P_.cleanup && ok && !primal_feasible()) {
+        // Dual feasible for the original costs, but dualSimplexInit moved boxed nonbasic
+        // variables whose reduced cost changed sign to their other bound: continue phase 2.
+        cleanups_++;
+        phase2();
+      }
     }
     pull();
     Z_ = hS_->obj;
@@ -909,6 +1053,10 @@ void Solver::result(ds_result& r) const {
   r.kernel_k = k_;
   r.nnz_nonbasic_mark = nnzN_mark_;
   r.nnz_nonbasic_end = nnzN_end_;
+  r.cleanups = cleanups_;
+  r.pad0 = 0;
+  for (int k = 0; k < DS_NPARTS; ++k) r.step_ms[k] = step_ms_[k];
+  if (P_.profile) r.step_ms[DS_PART_FACTOR] = refactor_ms_;
 }
 
 void Solver::solution(double& Z, int& solved, std::string& message, std::vector<double>& x) {

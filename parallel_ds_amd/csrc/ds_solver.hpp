@@ -92,6 +92,7 @@ class Solver {
                           double& bytes);
 
   std::string last_error;
+  hipStream_t stream() const { return st_; }
 
  private:
   int device_ = 0;
@@ -110,6 +111,12 @@ class Solver {
   DBuf<int> basis_, nonbasis_, posN_, eta_pos_, eta_last_, blocked_list_, list_idx_, ylist_row_, list_n_, part_i_,
       cand_i_;
   DBuf<unsigned char> blocked_;
+  // bound-flipping ratio test (allocated for models with boxed variables)
+  DBuf<unsigned char> bmark_;
+  DBuf<signed char> flipc_;
+  DBuf<int> blist_, flist_;
+  DBuf<double> bcc_, dxb_;
+  bool has_boxed_ = false;
   DBuf<DevScalars> S_;
   DevScalars* hS_ = nullptr;  // pinned mirror
   // factor
@@ -130,7 +137,16 @@ class Solver {
   bool solved_ = false;
   std::string message_;
   double Z_ = 0.0;
-  int refactors_ = 0, restarts_ = 0, phase1_iters_ = 0;
+  int refactors_ = 0, restarts_ = 0, phase1_iters_ = 0, cleanups_ = 0;
+  // per-step profile (params.profile): one event before the iteration and one after each step
+  double step_ms_[DS_NPARTS] = {};
+  std::vector<hipEvent_t> pev_;
+  std::vector<int> pev_part_;
+  void prof_event(int part);
+  void prof_harvest();
+  double prof_t0_ = 0.0;
+  void prof_host_begin();
+  void prof_host_end(int part);
   double solve_ms_ = 0.0, refactor_ms_ = 0.0;
   uint64_t rng_state_ = 0;
   // timing
@@ -163,6 +179,8 @@ class Solver {
   void upload_costs();
   int phase1();
   bool phase2();
+  bool dual_feasible();
+  bool primal_feasible();
   void enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t);
   void enqueue_stageB(int phase, int it, int t);
   int plan_chunk(int phase, int next, int t, int skip_first);

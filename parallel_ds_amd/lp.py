@@ -8,7 +8,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._native import DS_FIXED, DS_LOWER, DS_UPPER, lib, ptr
+from ._native import DS_BOXED, DS_FIXED, DS_LOWER, DS_UPPER, lib, ptr
 
 
 @dataclass
@@ -153,3 +153,16 @@ def _splitmix_first_int(seed, span):
     z = z ^ (z >> 31)
     u = (z >> 11) * (1.0 / 9007199254740992.0)
     return min(span - 1, int(u * span))
+
+
+def boxed_instance(m, n, density, seed):
+    """The small-LP recipe with boxed structurals and ranged rows: 0 <= x_j <= u_j,
+    u_j ~ U(0.3, 1), and 0.2 b_i <= (A x)_i <= b_i (a Boxed slack per row).  With c < 0 most
+    structurals end at their upper bound, so the dual simplex flips bounds throughout."""
+    base = gen_bernoulli(m, n, density, seed)
+    rng = np.random.default_rng(seed + 7919)
+    ub = rng.uniform(0.3, 1.0, n)
+    b = base.upper_range
+    return LP(m, n, base.col_ptr, base.row_idx, base.val, base.costs, np.zeros(n), ub,
+              np.full(n, DS_BOXED, np.int32), 0.2 * b, b.copy(), np.full(m, DS_BOXED, np.int32),
+              "boxed-m%d-n%d-s%d" % (m, n, seed))

@@ -133,3 +133,16 @@ def harris_ratio_test(F, d_of_pos, alpha):
     L.ora_harris_ratio_test.restype = C.c_int
     L.ora_harris_ratio_test.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     return L.ora_harris_ratio_test(len(F), P(F), P(d_of_pos), P(alpha))
+
+
+def dse_pricing(bt, x, l, u, beta):
+    """DSEPricing over arrays by basis position: (p_idx or -1, delta)."""
+    bt = np.ascontiguousarray(bt, np.int8)
+    x, l, u, beta = [np.ascontiguousarray(a, np.float64) for a in (x, l, u, beta)]
+    p = C.c_int()
+    d = C.c_double()
+    L = lib()
+    L.ora_dse_pricing.restype = C.c_int
+    L.ora_dse_pricing.argtypes = [C.c_int] + [C.c_void_p] * 5 + [C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.ora_dse_pricing(len(x), P(bt), P(x), P(l), P(u), P(beta), C.byref(p), C.byref(d))
+    return p.value, d.value

@@ -33,12 +33,13 @@ def mixed_lp(seed, boxed=False, m_max=5, n_max=6, free_upper=True):
 
 
 def highs(L):
-    """(status, objective) from scipy's HiGHS: 0 optimal, 1 infeasible, 3 unbounded."""
+    """(status, objective) from scipy's HiGHS: 0 optimal, 1 infeasible, 3 unbounded.  Row range types:
+    0 equality, 3 <= upper_range, 4 >= lower_range, 2 both (a ranged row)."""
     from scipy.optimize import linprog
     A = sp.csc_matrix((L.val, L.row_idx, L.col_ptr), shape=(L.m, L.n)).toarray()
     eq = L.range_type == 0
-    le = L.range_type == 3
-    ge = L.range_type == 4
+    le = (L.range_type == 3) | (L.range_type == 2)
+    ge = (L.range_type == 4) | (L.range_type == 2)
     A_ub = np.vstack([A[le], -A[ge]])
     b_ub = np.concatenate([L.upper_range[le], -L.lower_range[ge]])
     bounds = [(None if not np.isfinite(a) else a, None if not np.isfinite(b) else b)

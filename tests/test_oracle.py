@@ -356,3 +356,50 @@ def test_solve_from_a_given_basis(seed):
     res2, x2, b2 = ora.solve_from(L, cols.astype(np.int32))
     assert res2.status == 0
     assert abs(res2.Z - g["objective"]) <= 1e-9 * abs(g["objective"])
+
+
+@pytest.mark.parametrize("m,n,density", [(50, 100, 0.1), (200, 400, 0.05), (1000, 2000, 0.01)])
+def test_boxed_family_against_highs(m, n, density):
+    """Boxed structurals and ranged rows (every slack Boxed): most structurals end at their
+    upper bound, reached through bound flips; the oracle's objective equals HiGHS' and its
+    solution is certified. The 1000 x 2000 instances perturb their costs: after the costs are
+    restored the cleanup places the boxed nonbasic variables by the restored reduced costs and
+    resumes phase 2 (without it, seed 0 ends 3e-9 above HiGHS with two boxed variables at the
+    wrong bound)."""
+    import mixed_lps
+    for s in range(4 if m < 1000 else 2):
+        L = lpmod.boxed_instance(m, n, density, s)
+        res, x, basis = ora.solve(L, tie_rule=1, cleanup=1)
+        hs, hz = mixed_lps.highs(L)
+        assert res.status == 0 and hs == 0 and res.bfrt_exhausted == 0
+        assert abs(res.Z - hz) <= 1e-9 * abs(hz)
+        pinf, dinf, _ = lpcheck.certificate(L, x, basis)
+        assert pinf <= 1e-7 and dinf <= 1e-7
+        l, u, _ = lpcheck.comp_bounds(L)
+        assert np.sum(np.abs(x[:n] - u[:n]) < 1e-9) > n // 2
+
+
+def test_dse_pricing_known_answers():
+    """ora_dse_pricing against hand-derived cases of DSEPricing + checkVarIsInfeas
+    (baseDualSimplex.tpp:443-501): the weight delta^2/beta, the 1e-8 floor, the strict
+    comparison over ascending positions (ties to the lowest), and the case fallthrough
+    (a Lower variable within 1e-10 of its bound falls into the Fixed test and gives delta ~ 0,
+    which the floor discards; an Upper variable below its lower bound is never flagged)."""
+    inf = np.inf
+    LOWER, UPPER, BOXED, FIXED, FREE = 4, 3, 2, 0, 1
+    # (bt, x, l, u, beta) rows -> expected (p_idx, delta)
+    cases = [
+        ([LOWER, LOWER], [-1.0, -2.0], [0, 0], [inf, inf], [1.0, 1.0], (1, -2.0)),
+        ([LOWER, LOWER], [-1.0, -2.0], [0, 0], [inf, inf], [1.0, 5.0], (0, -1.0)),   # 1 > 4/5
+        ([LOWER, LOWER], [-2.0, -2.0], [0, 0], [inf, inf], [1.0, 1.0], (0, -2.0)),   # tie -> lowest
+        ([BOXED, UPPER], [3.0, 0.5], [0, -inf], [1.0, 0.0], [4.0, 0.25], (0, 2.0)),  # 4/4 = 1 = .25/.25
+        ([UPPER], [-5.0], [-inf], [0.0], [1.0], (-1, 0.0)),                          # below l: not flagged
+        ([FIXED], [1.0 + 1e-11], [1.0], [1.0], [1.0], (-1, 0.0)),                   # within tol, floor
+        ([LOWER], [0.5e-10], [0.0], [inf], [1.0], (-1, 0.0)),                        # fallthrough to Fixed
+        ([FIXED], [1.5], [1.0], [1.0], [2.0], (0, 0.5)),                             # via the Boxed case
+        ([FREE], [-7.0], [-inf], [inf], [1.0], (-1, 0.0)),
+        ([LOWER], [-1e-9], [0.0], [inf], [1.0], (-1, 0.0)),                          # |delta| below 1e-8
+        ([LOWER], [-2e-8], [0.0], [inf], [1.0], (0, -2e-8)),
+    ]
+    for bt, x, l, u, beta, want in cases:
+        assert ora.dse_pricing(bt, x, l, u, beta) == want, (bt, x, want)
