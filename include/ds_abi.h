@@ -158,6 +158,19 @@ int ds_get_nonbasis(ds_ctx* ctx, int* nonbasis /* n */);
 int ds_chuzr(ds_ctx* ctx, int m, const signed char* bt_B, const double* x_B, const double* l_B, const double* u_B,
              const double* beta, int* p_idx, double* delta);
 
+/* ds_chuzr without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as ds_chuzr; its scratch (per-workgroup partials and an arrival
+ * counter, 8 KiB) is a static device array that each GPU holds until the
+ * process ends, set up by ds_create.  m above DS_CHUZR_NOALLOC_MAX_M is
+ * rejected with DS_EINVAL.  Calls that overlap on one GPU (from several
+ * contexts or threads) share that scratch and race; the counter is cleared at
+ * the start of every call. */
+#define DS_CHUZR_NOALLOC_MAX_M (1 << 28)
+int ds_chuzr_noalloc(ds_ctx* ctx, int m, const signed char* bt_B, const double* x_B, const double* l_B,
+                     const double* u_B, const double* beta, int* p_idx, double* delta);
+
 /* Timing of the pivot-row SpMV on the context's current basis and a given rho:
  * `reps` launches, HIP events on the context stream; returns the mean ms per
  * launch in *ms and the algorithmic bytes of one launch in *bytes. */

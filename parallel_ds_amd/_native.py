@@ -70,6 +70,7 @@ _SIGS = {
     "ds_get_basis": (I32, [VP, VP]),
     "ds_get_nonbasis": (I32, [VP, VP]),
     "ds_chuzr": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP]),
+    "ds_chuzr_noalloc": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP]),
     "ds_solve_lp": (I32, [VP, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.POINTER(Params),
                           C.POINTER(Result), VP, VP]),
     "ds_set_basis": (I32, [VP, VP]),

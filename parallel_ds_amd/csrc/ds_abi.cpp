@@ -12,6 +12,10 @@
 
 #include "ds_ctx.hpp"
 
+namespace pds {
+int chuzr_noalloc_setup(int device);  // ds_chuzr.hip
+}
+
 struct ds_mlp {
   pds::Mlp* net = nullptr;
   std::string err;
@@ -106,9 +110,14 @@ int ds_create(int device, ds_ctx** out) {
   if (device < 0 || device >= ndev) return DS_EINVAL;
   ds_ctx* c = new ds_ctx();
   try {
-    c->solver = new pds::Solver(device);
+    c->solver = new pds::Solver(device);  // makes `device` current
     c->device = device;
   } catch (const std::exception& e) {
+    delete c;
+    return DS_EHIP;
+  }
+  if (pds::chuzr_noalloc_setup(device) != DS_OK) {
+    delete c->solver;
     delete c;
     return DS_EHIP;
   }

@@ -5,6 +5,7 @@
 // serves a caller that keeps its own basis data in HBM.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "ds_abi.h"
@@ -103,6 +104,78 @@ extern "C" int ds_chuzr(ds_ctx* ctx, int m, const signed char* bt_B, const doubl
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     ctx->err = std::string("ds_chuzr: ") + hipGetErrorString(e);
+    return DS_EHIP;
+  }
+  return DS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant.  Its scratch (the partials of CZ_GRID_MAX
+// workgroups and the arrival counter) is a file-scope device array; every GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol): chuzr_noalloc_setup() looks up the device address once per
+// device, with that device current, and the entry passes that address.
+// ---------------------------------------------------------------------------
+namespace pds {
+namespace {
+
+constexpr size_t CZ_SCRATCH_BYTES = 8192;  // 512 doubles + 513 ints, rounded up
+static_assert(CZ_GRID_MAX * (sizeof(double) + sizeof(int)) + sizeof(int) <= CZ_SCRATCH_BYTES, "scratch too small");
+
+static __device__ __align__(256) unsigned char scratch[CZ_SCRATCH_BYTES];
+
+constexpr int CZ_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[CZ_MAX_DEVICES] = {};
+
+}  // namespace
+
+// Called by ds_create with `device` current.
+int chuzr_noalloc_setup(int device) {
+  if (device < 0 || device >= CZ_MAX_DEVICES) return DS_EINVAL;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DS_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DS_EHIP;
+  g_scratch_ptr[device] = p;
+  return DS_OK;
+}
+
+}  // namespace pds
+
+extern "C" int ds_chuzr_noalloc(ds_ctx* ctx, int m, const signed char* bt_B, const double* x_B, const double* l_B,
+                                const double* u_B, const double* beta, int* p_idx, double* delta) {
+  if (!ctx) return DS_EINVAL;
+  if (!chuzr_args_ok(m, bt_B, x_B, l_B, u_B, beta, p_idx, delta)) {
+    ctx->err = "ds_chuzr_noalloc: m must be positive and every pointer set";
+    return DS_EINVAL;
+  }
+  if (m > DS_CHUZR_NOALLOC_MAX_M) {
+    ctx->err = "ds_chuzr_noalloc: m = " + std::to_string(m) + " exceeds DS_CHUZR_NOALLOC_MAX_M = " +
+               std::to_string(DS_CHUZR_NOALLOC_MAX_M);
+    return DS_EINVAL;
+  }
+  void* base = nullptr;
+  if (ctx->device >= 0 && ctx->device < CZ_MAX_DEVICES) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) {
+    ctx->err = "ds_chuzr_noalloc: scratch address of this device was not set up by ds_create";
+    return DS_ESTATE;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return DS_EHIP;
+  double* part_w = (double*)base;
+  int* part_i = (int*)((unsigned char*)base + sizeof(double) * CZ_GRID_MAX);
+  int* ctr = part_i + CZ_GRID_MAX;
+  hipStream_t s = ctx->solver->stream();
+  k_chuzr_reset<<<1, 1, 0, s>>>(ctr);
+  k_chuzr<<<chuzr_grid(m), CZ_BS, 0, s>>>(m, bt_B, x_B, l_B, u_B, beta, part_w, part_i, ctr, p_idx, delta);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    ctx->err = std::string("ds_chuzr_noalloc: ") + hipGetErrorString(e);
     return DS_EHIP;
   }
   return DS_OK;
