@@ -76,7 +76,21 @@ typedef struct {
                              sequentialDualSimplex.cpp:113-116, so that basis is returned) */
   int profile;            /* 1: per-step device times in ds_result.step_ms (HIP events between the steps
                              of every iteration, one iteration per host synchronisation) */
+  int force_event;        /* test hook, 0 = off: force one recovery path at iteration force_at:
+                             DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART or DS_FORCE_SINGULAR */
+  int force_at;
 } ds_params;
+
+/* Recovery paths a test can force from the host (sequentialDualSimplex.cpp):
+ *  STABILITY     the stability check fails (|alpha_q[p] - alpha_r[q]| test, :428-433): the
+ *                recorded alpha_r[q] is moved by 1e-6 relative after CHUZC, so the basis is
+ *                refactored before the update, which then uses the exact alpha_q;
+ *  DUAL_RESTART  at the next iteration that is a multiple of 40 from force_at on, one nonbasic
+ *                Lower variable's reduced cost is made negative before the periodic dual check
+ *                (:285-293), which then refactors, reruns phase 1 and re-initialises;
+ *  SINGULAR      the first refactorization at or after force_at sees one basic structural
+ *                column as zero: the column is replaced by a slack and the iterate recomputed. */
+enum { DS_FORCE_NONE = 0, DS_FORCE_STABILITY = 1, DS_FORCE_DUAL_RESTART = 2, DS_FORCE_SINGULAR = 3 };
 
 /* Steps of ds_result.step_ms, the parts of the reference's TimeProfiler
  * (common/types.hpp:12-27, common/timeProfiler.cpp:22-148).  The GPU fuses
@@ -107,7 +121,7 @@ typedef struct {
   double nnz_nonbasic_mark; /* nnz of the nonbasic columns at the mark and at the end */
   double nnz_nonbasic_end;
   int cleanups;           /* dual simplex resumptions after restoring perturbed costs */
-  int pad0;
+  int repairs;            /* structural columns replaced by slacks in singular factorizations */
   double step_ms[DS_NPARTS]; /* with params.profile: time per step (DS_PART_*) */
 } ds_result;
 
@@ -124,6 +138,18 @@ int ds_load_problem(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* ro
                     const int* range_type);
 int ds_set_params(ds_ctx* ctx, const ds_params* p);
 int ds_init(ds_ctx* ctx);
+/* ds_init from a given starting basis (m variable ids; the nonbasic set is the
+ * ascending complement) instead of the slack basis.  No counterpart in the
+ * reference (its solver always starts from the slack basis,
+ * sequentialDualSimplex.cpp:230-250); the algorithm that follows is the same,
+ * with exact DSE weights of the starting basis (initBetaWeights(false),
+ * baseDualSimplex.tpp:41-68).  A column of the basis that the factorization
+ * finds dependent (pivot at or below 1e-13 of its row's scale) is replaced by
+ * the slack of the row chosen for it; ds_result.repairs counts them.  The same
+ * substitution keeps a solve going when a refactorization inside it meets a
+ * numerically singular basis; the iterate is then recomputed from the new
+ * basis.  ds_set_basis (hook level) still fails with DS_ESINGULAR. */
+int ds_init_basis(ds_ctx* ctx, const int* basis);
 int ds_presolve(ds_ctx* ctx, const char* method, int* phase1_status); /* 0 Solved, 1 DualInfeas */
 int ds_solve(ds_ctx* ctx);
 int ds_get_result(ds_ctx* ctx, ds_result* r);
@@ -211,6 +237,11 @@ int ds_batch_load(ds_ctx* ctx, int count, const int* m, const int* n, const long
                   const int* row_idx, const double* val, const double* b, const double* c);
 /* Starting bases (m_k variable ids per LP, concatenated); NULL restores the slack basis. */
 int ds_batch_set_warm(ds_ctx* ctx, const int* basis);
+/* Bases of the batch's LPs (m_k variable ids per LP, concatenated) found by
+ * another solver, e.g. ds_solve for LPs beyond the batched kernel's LDS limit:
+ * they take the place of a batched solve's bases as the labels that
+ * ds_batch_labels and ds_mlp_fit read. */
+int ds_batch_set_bases(ds_ctx* ctx, const int* basis);
 int ds_batch_solve(ds_ctx* ctx, const ds_batch_params* p, double* ms);
 int ds_batch_replay(ds_ctx* ctx, double* ms);  /* relaunch the captured graph */
 int ds_batch_results(ds_ctx* ctx, int* status, double* obj, int* iters, int* basis /* sum m */,

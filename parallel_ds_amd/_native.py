@@ -19,7 +19,10 @@ class Params(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
                 ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
                 ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
-                ("profile", C.c_int)]
+                ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int)]
+
+
+DS_FORCE_NONE, DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART, DS_FORCE_SINGULAR = 0, 1, 2, 3
 
 
 # Steps of Result.step_ms (the reference's TimeProfiler parts, common/types.hpp:12-27).
@@ -33,7 +36,7 @@ class Result(C.Structure):
                 ("restarts", C.c_int), ("Z", C.c_double), ("solve_ms", C.c_double), ("refactor_ms", C.c_double),
                 ("mark_ms", C.c_double), ("spmv_ms", C.c_double), ("spmv_launches", C.c_int), ("kernel_k", C.c_int),
                 ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double), ("cleanups", C.c_int),
-                ("pad0", C.c_int), ("step_ms", C.c_double * 13)]
+                ("repairs", C.c_int), ("step_ms", C.c_double * 13)]
 
 
 class BatchParams(C.Structure):
@@ -63,6 +66,7 @@ _SIGS = {
     "ds_load_problem": (I32, [VP, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "ds_set_params": (I32, [VP, C.POINTER(Params)]),
     "ds_init": (I32, [VP]),
+    "ds_init_basis": (I32, [VP, VP]),
     "ds_presolve": (I32, [VP, C.c_char_p, C.POINTER(I32)]),
     "ds_solve": (I32, [VP]),
     "ds_get_result": (I32, [VP, C.POINTER(Result)]),
@@ -86,6 +90,7 @@ _SIGS = {
     "ds_batch_default_params": (None, [C.POINTER(BatchParams)]),
     "ds_batch_load": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP]),
     "ds_batch_set_warm": (I32, [VP, VP]),
+    "ds_batch_set_bases": (I32, [VP, VP]),
     "ds_batch_solve": (I32, [VP, C.POINTER(BatchParams), C.POINTER(F64)]),
     "ds_batch_replay": (I32, [VP, C.POINTER(F64)]),
     "ds_batch_results": (I32, [VP, VP, VP, VP, VP, VP]),

@@ -71,6 +71,12 @@ class BatchSolver:
         flat = np.concatenate([np.asarray(b, np.int32) for b in bases]).astype(np.int32)
         self._check(lib().ds_batch_set_warm(self._ctx, ptr(flat)), "ds_batch_set_warm")
 
+    def set_bases(self, bases):
+        """Bases found by another solver (list of m_k-length variable-id arrays): the labels of
+        the learned warm start, in place of a batched solve."""
+        flat = np.concatenate([np.asarray(b, np.int32) for b in bases]).astype(np.int32)
+        self._check(lib().ds_batch_set_bases(self._ctx, ptr(flat)), "ds_batch_set_bases")
+
     def solve(self, params=None):
         p = params if params is not None else default_batch_params()
         ms = C.c_double()

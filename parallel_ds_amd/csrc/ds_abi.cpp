@@ -100,6 +100,8 @@ void ds_default_params(ds_params* p) {
   p->time_kernels = 0;
   p->cleanup = 1;
   p->profile = 0;
+  p->force_event = DS_FORCE_NONE;
+  p->force_at = 0;
 }
 
 int ds_create(int device, ds_ctx** out) {
@@ -167,6 +169,11 @@ int ds_set_params(ds_ctx* ctx, const ds_params* p) {
 
 int ds_init(ds_ctx* ctx) {
   return guard(ctx, [&](pds::Solver& s) { s.initDualSimplex(); });
+}
+
+int ds_init_basis(ds_ctx* ctx, const int* basis) {
+  if (!basis) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.init_basis(basis); });
 }
 
 int ds_presolve(ds_ctx* ctx, const char* method, int* phase1_status) {
@@ -298,6 +305,11 @@ int ds_batch_load(ds_ctx* ctx, int count, const int* m, const int* n, const long
 
 int ds_batch_set_warm(ds_ctx* ctx, const int* basis) {
   return bguard(ctx, [&](pds::BatchSolver& B) { B.set_warm(basis); });
+}
+
+int ds_batch_set_bases(ds_ctx* ctx, const int* basis) {
+  if (!basis) return DS_EINVAL;
+  return bguard(ctx, [&](pds::BatchSolver& B) { B.set_bases(basis); });
 }
 
 int ds_batch_solve(ds_ctx* ctx, const ds_batch_params* p, double* ms) {

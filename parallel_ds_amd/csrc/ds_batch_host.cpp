@@ -177,6 +177,27 @@ void BatchSolver::set_warm(const int* basis) {
   drop_graph();
 }
 
+// Bases found elsewhere (for instance by the single-LP solver, for LPs too large
+// for the batched kernel) stand in for a solve of the batch: they become the
+// labels of the learned warm start.
+void BatchSolver::set_bases(const int* basis) {
+  std::vector<int> w(basis, basis + sum_m_);
+  long long o = 0;
+  for (int k = 0; k < count_; ++k) {
+    const int N = hm_[k] + hn_[k];
+    std::vector<char> seen(N, 0);
+    for (int i = 0; i < hm_[k]; ++i) {
+      int j = w[o + i];
+      if (j < 0 || j >= N || seen[j]) throw std::invalid_argument("invalid basis for LP " + std::to_string(k));
+      seen[j] = 1;
+    }
+    o += hm_[k];
+  }
+  up(basis_, w, st_);
+  PDS_HIP(hipStreamSynchronize(st_));
+  solved_ = true;
+}
+
 // Build the per-class launch descriptions for the LP subset `items`.
 void BatchSolver::plan(const std::vector<int>& items, const ds_batch_params& p) {
   std::vector<int> small, large;

@@ -15,6 +15,7 @@ class BatchSolver {
   void load(int count, const int* m, const int* n, const long long* nnz_off, const int* col_ptr, const int* row_idx,
             const double* val, const double* b, const double* c);
   void set_warm(const int* basis);
+  void set_bases(const int* basis);
   double solve(const ds_batch_params& p);
   double replay();
   void results(int* status, double* obj, int* iters, int* basis, double* x);

@@ -1207,6 +1207,34 @@ __global__ __launch_bounds__(BS) void k_bfrt_xb(DevModel M, DevIter I) {
 }
 
 // ---------------------------------------------------------------------------
+// Test hooks that force a recovery path from the host; every launch ends
+// normally.
+// ---------------------------------------------------------------------------
+// The recorded alpha_r[q] moves by 1e-6 relative: the stability check fails.
+__global__ void k_force_stability(DevIter I) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  S->alpha_rq = S->alpha_rq + 1e-6 * (1.0 + fabs(S->alpha_rq));
+}
+// The first nonbasic Lower variable with d_j > EPS_D gets d_j = -1: the basis is
+// no longer dual feasible.
+__global__ void k_force_dual(DevModel M, DevIter I) {
+  if (stopped(I.S)) return;
+  for (int i = 0; i < M.N - M.m; ++i) {
+    const int j = I.nonbasis[i];
+    if (M.bt[j] == BT_LOWER && I.d[j] > EPS_D) {
+      I.d[j] = -1.0;
+      return;
+    }
+  }
+}
+// Kernel column c of the work matrix reads zero: exactly dependent.
+__global__ void k_force_zero_column(double* W, int k, int c) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < k) W[(size_t)i * k + c] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // Initialisation helpers.
 // ---------------------------------------------------------------------------
 
@@ -1483,6 +1511,13 @@ void launch_flips(hipStream_t s, const DevModel& M, const DevIter& I, const DevF
   launch_ftran_dense(s, M, I, F, I.bcc, I.dxb, t, 0, 2);
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   k_bfrt_xb<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I);
+}
+void launch_force(hipStream_t s, const DevModel& M, const DevIter& I, int event) {
+  if (event == DS_FORCE_STABILITY) k_force_stability<<<1, 1, 0, s>>>(I);
+  else if (event == DS_FORCE_DUAL_RESTART) k_force_dual<<<1, 1, 0, s>>>(M, I);
+}
+void launch_force_zero_column(hipStream_t s, double* W, int k, int c) {
+  if (k > 0 && c >= 0 && c < k) k_force_zero_column<<<(k + 255) / 256, 256, 0, s>>>(W, k, c);
 }
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I) {
   k_set_primal<<<nblk(M.N - M.m, BS), BS, 0, s>>>(M, I);

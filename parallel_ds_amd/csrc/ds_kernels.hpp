@@ -1,5 +1,6 @@
 // Host-callable launchers of the gfx950 kernels (asynchronous on `s`).
 #pragma once
+#include "ds_abi.h"
 #include "ds_dev.hpp"
 
 namespace pds {
@@ -27,6 +28,10 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
 // column change, its FTRAN and the x_B / objective correction.
 void launch_bfrt(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int harris, int t);
 void launch_flips(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t);
+// Test hooks (ds_params.force_event): DS_FORCE_STABILITY / DS_FORCE_DUAL_RESTART in the
+// iteration chain, and a zeroed kernel column before a factorization.
+void launch_force(hipStream_t s, const DevModel& M, const DevIter& I, int event);
+void launch_force_zero_column(hipStream_t s, double* W, int k, int c);
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I);
 void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN);
 void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,
@@ -43,7 +48,7 @@ struct GJWork {
   double* Pb;     // k x NB saved panel
   int* piv;       // k
   int* perm;      // k
-  int* flag;      // singular-pivot flag
+  int* flag;      // [0] singular flag, [1] count of dependent steps, [2..] those steps (GJ_DEP_CAP)
   double* Pinv;   // NB x NB inverse of the pivot block
   int* rowsA;     // tournament candidate lists (ceil(k/256)+1) x NB, two buffers
   int* rowsB;
@@ -53,6 +58,7 @@ struct GJWork {
   double* rowscale;  // k: largest |entry| of each row of K, kept in step with the row swaps
 };
 constexpr int GJ_NB = 64;
+constexpr int GJ_DEP_CAP = 4096;  // dependent kernel columns one factorization can report
 void launch_build_kernel(hipStream_t s, const DevModel& M, const int* basis, const DevFactor& F, double* W);
 void launch_gj_panels(hipStream_t s, int k, const GJWork& G);
 void launch_gj_unscramble(hipStream_t s, int k, const GJWork& G, double* Kinv, double* KinvT);

@@ -50,6 +50,20 @@ __global__ __launch_bounds__(256) void k_build_kernel(DevModel M, const int* bas
 constexpr int TS_ROWS = 256;  // rows per group of the multi-round tournament (4 waves)
 constexpr int TS_BLK = 16;
 constexpr double SING_REL = 1e-13;  // relative pivot below which the kernel counts as singular
+
+// A final-round pivot step without a usable pivot: the singular flag, and the
+// step (kernel column) with the row the tournament chose for it (its position
+// in W before this panel's exchanges), which the host uses to substitute that
+// row's slack for the column.
+__device__ inline void mark_dependent(int* flag, int step, int row) {
+  flag[0] = 1;
+  const int c = flag[1];
+  if (c < GJ_DEP_CAP) {
+    flag[2 + 2 * c] = step;
+    flag[3 + 2 * c] = row;
+  }
+  flag[1] = c + 1;
+}
 static_assert(NB == WAVE, "one pivot-block row per lane");
 static_assert(NB % TS_BLK == 0, "unrolled blocks tile the panel");
 
@@ -108,6 +122,7 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
   __shared__ double cs[2][TS_WAVES];  // the candidate row's scale (largest |entry| of its panel row, as loaded)
   __shared__ int win[NB];
   __shared__ int wrow[NB];
+  __shared__ unsigned char depf[NB];  // final round: the step had no usable pivot
   __shared__ double tile[TILE ? ROWS : 1][NB + 1];  // the group's rows of the panel, as loaded
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
@@ -195,17 +210,21 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
         // steps <= R, so some row is live and gr is a row of this group.
         if (tid == 0) win[st] = gr;
         if (tid == gr) live = false;
-        // Final round: a pivot at or below SING_REL of its row's scale in K marks
-        // the kernel singular (rounding leaves dependent rows near, not at, zero).
-        if (final_ && tid == 0 && gv > 0.0 && gv <= SING_REL * cs[b][gr / WAVE]) *flag = 1;
-        if (gv > 0.0) {
+        // Final round: a pivot at or below SING_REL of its row's scale in K (or no
+        // nonzero pivot) marks the column dependent: rounding leaves dependent rows
+        // near, not at, zero.  The step then proceeds as if the column were the
+        // unit vector of the chosen row (multipliers 0), so that later columns are
+        // judged on the remaining rows and not on rounding noise.
+        const bool dep = final_ && (gv <= 0.0 || gv <= SING_REL * cs[b][gr / WAVE]);
+        if (tid == 0) {
+          depf[st] = dep ? 1 : 0;
+          if (dep) mark_dependent(flag, j + st, rid[gr]);
+        }
+        if (gv > 0.0 && !dep) {
           const double* pr = cand[b][gr / WAVE];
           const double l = a[s] * recip_nr(pr[s]);
 #pragma unroll
           for (int x = s + 1; x < NB; ++x) a[x] = a[x] - l * pr[x];
-        } else if (final_ && tid == 0) {
-          // No nonzero pivot in this column: only the final round makes it an error.
-          *flag = 1;
         }
       }
     }
@@ -237,7 +256,8 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
       const int c = GJ_COLS * wv + x;
       double v = 0.0;
       if (lane < steps) {
-        if (c < NB) {
+        if (c < NB && c < steps && depf[c]) v = (c == lane) ? 1.0 : 0.0;  // dependent: unit column
+        else if (c < NB) {
           if constexpr (TILE) v = tile[sl][c];
           else v = c < nb ? W[(size_t)rid[sl] * k + j + c] : 0.0;
         }
@@ -293,6 +313,21 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
       if (lane == loc - j) oc = occ;
       if (lane == s2) oc = myrow;
     }
+  }
+}
+
+// Dependent columns of panel j (final round): column step := unit vector of its
+// chosen row in W (before the panel's exchanges), matching the tournament's
+// treatment, so the trailing update leaves the other columns as they would be
+// for a basis with that row's slack in place of the column.
+__global__ __launch_bounds__(256) void k_dep_columns(double* W, int k, int j, int nb, const int* flag) {
+  const int n = min(flag[1], GJ_DEP_CAP);
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= k) return;
+  for (int d = 0; d < n; ++d) {
+    const int step = flag[2 + 2 * d];
+    if (step < j || step >= j + nb) continue;
+    W[(size_t)i * k + step] = (i == flag[3 + 2 * d]) ? 1.0 : 0.0;
   }
 }
 
@@ -507,6 +542,7 @@ void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
       out_n = tn;
       groups = g2;
     }
+    k_dep_columns<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.flag);
     k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv, G.rowscale);
     k_panel_save<<<(int)(((long)k * nb + 255) / 256), 256, 0, s>>>(G.W, k, j, nb, G.Pb);
     k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv);

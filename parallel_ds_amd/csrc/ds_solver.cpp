@@ -358,7 +358,7 @@ void Solver::load(const HostModel& hm) {
   piv_.alloc(m);
   rowscale_.alloc(m);
   perm_.alloc(m);
-  flag_.alloc(1);
+  flag_.alloc(2 + 2 * GJ_DEP_CAP);
   Pb_.alloc((size_t)m * GJ_NB);
   Pinv_.alloc((size_t)GJ_NB * GJ_NB);
   rowsA_.alloc((size_t)(m / 256 + 2) * GJ_NB);
@@ -393,6 +393,49 @@ static void ensure_kernel_capacity(DBuf<double>& b, int k) {
 
 void Solver::refactor() {
   const double t0 = now_ms();
+  // A numerically singular kernel either ends the call (repair off) or has the
+  // slacks of the rows chosen for its dependent columns substituted for those
+  // columns, after which the basis is factored again.
+  for (int round = 0;; ++round) {
+    std::vector<std::pair<int, int>> dep;
+    factor_once(dep);
+    if (dep.empty()) break;
+    if (!repair_ || round >= 8) throw SingularError("singular basis kernel in refactorization");
+    substitute_slacks(dep);
+  }
+  refactors_++;
+  refactor_ms_ += now_ms() - t0;
+}
+
+// Basis position pos_P[g] of each dependent kernel column g takes the slack of
+// the original row whose kernel row the tournament chose for that step; the
+// leaving structural takes the slack's place in the nonbasic list.
+void Solver::substitute_slacks(const std::vector<std::pair<int, int>>& dep) {
+  const int m = H_.m, n = H_.n, N = H_.N;
+  std::vector<int> basis = basis_host(), nonbasis = nonbasis_host(), posN(N);
+  PDS_HIP(hipMemcpyAsync(posN.data(), posN_.p, sizeof(int) * N, hipMemcpyDeviceToHost, st_));
+  sync();
+  for (const auto& d : dep) {
+    const int pos = fac_pos_P_[d.first];
+    const int in = n + fac_row_R_[d.second];
+    const int out = basis[pos];
+    const int q = posN[in];
+    if (q < 0 || out >= n) throw SingularError("slack substitution met an inconsistent basis");
+    basis[pos] = in;
+    nonbasis[q] = out;
+    posN[out] = q;
+    posN[in] = -1;
+  }
+  (void)m;
+  upload(basis_, basis, st_);
+  upload(nonbasis_, nonbasis, st_);
+  upload(posN_, posN, st_);
+  sync();
+  repairs_ += (int)dep.size();
+  repaired_ = true;
+}
+
+void Solver::factor_once(std::vector<std::pair<int, int>>& dep) {
   const int m = H_.m, n = H_.n;
   std::vector<int> basis(m);
   PDS_HIP(hipMemcpyAsync(basis.data(), basis_.p, sizeof(int) * m, hipMemcpyDeviceToHost, st_));
@@ -465,17 +508,41 @@ void Solver::refactor() {
     ensure_kernel_capacity(GW_, k);
     ensure_kernel_capacity(Kinv_, k);
     ensure_kernel_capacity(KinvT_, k);
-    int zero = 0;
-    PDS_HIP(hipMemcpyAsync(flag_.p, &zero, sizeof(int), hipMemcpyHostToDevice, st_));
+    const int zero2[2] = {0, 0};
+    PDS_HIP(hipMemcpyAsync(flag_.p, zero2, sizeof(zero2), hipMemcpyHostToDevice, st_));
     launch_build_kernel(st_, M(), basis_.p, F(), GW_.p);
+    if (P_.force_event == DS_FORCE_SINGULAR && !force_fired_ && repair_ && iteration_ >= P_.force_at) {
+      launch_force_zero_column(st_, GW_.p, k, k / 2);
+      force_fired_ = true;
+    }
     GJWork G{GW_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_, rowscale_.p};
     launch_gj_panels(st_, k, G);
-    int flag = 0;
+    int flag[2] = {0, 0};
     std::vector<int> piv(k), perm(k);
-    PDS_HIP(hipMemcpyAsync(&flag, flag_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+    PDS_HIP(hipMemcpyAsync(flag, flag_.p, sizeof(flag), hipMemcpyDeviceToHost, st_));
     PDS_HIP(hipMemcpyAsync(piv.data(), piv_.p, sizeof(int) * k, hipMemcpyDeviceToHost, st_));
     sync();
-    if (flag) throw SingularError("singular basis kernel in refactorization");
+    if (flag[0]) {
+      // Dependent steps (step, W row before that panel's exchanges) -> (kernel column,
+      // original kernel row), by replaying the row exchanges of the earlier panels.
+      if (flag[1] <= 0 || flag[1] > GJ_DEP_CAP) throw SingularError("singular basis kernel in refactorization");
+      std::vector<int> ent(2 * flag[1]);
+      PDS_HIP(hipMemcpyAsync(ent.data(), flag_.p + 2, sizeof(int) * ent.size(), hipMemcpyDeviceToHost, st_));
+      sync();
+      std::vector<int> pos2orig(k);
+      for (int r = 0; r < k; ++r) pos2orig[r] = r;
+      std::vector<std::pair<int, int>> out;
+      for (int j0 = 0; j0 < k; j0 += GJ_NB) {
+        const int j1 = std::min(k, j0 + GJ_NB);
+        for (int d = 0; d < flag[1]; ++d)
+          if (ent[2 * d] >= j0 && ent[2 * d] < j1) out.push_back({ent[2 * d], pos2orig[ent[2 * d + 1]]});
+        for (int g = j0; g < j1; ++g) std::swap(pos2orig[g], pos2orig[piv[g]]);
+      }
+      fac_pos_P_ = pos_P;
+      fac_row_R_ = row_R;
+      dep = out;
+      return;
+    }
     // Undo the row exchanges on the columns: composed swaps in reverse order.
     for (int c = 0; c < k; ++c) perm[c] = c;
     for (int s2 = k - 1; s2 >= 0; --s2) {
@@ -491,8 +558,6 @@ void Solver::refactor() {
   hS_->k = k;
   push();
   t_ = 0;
-  refactors_++;
-  refactor_ms_ += now_ms() - t0;
 }
 
 void Solver::initBetaWeights(bool identity) {
@@ -601,7 +666,12 @@ void Solver::initDualSimplex() {
   rng_state_ = 0;
   iteration_ = 0;
   t_ = 0;
-  refactors_ = restarts_ = phase1_iters_ = cleanups_ = 0;
+  refactors_ = restarts_ = phase1_iters_ = cleanups_ = repairs_ = 0;
+  warm_ = false;
+  repaired_ = false;
+  repair_ = false;
+  force_fired_ = false;
+  force_enq_it_ = -1;
   for (double& v : step_ms_) v = 0.0;
   pev_part_.clear();
   solve_ms_ = refactor_ms_ = 0.0;
@@ -622,7 +692,10 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   const DevIter i = I();
   const DevFactor f = F();
   prof_event(DS_PART_PRICING);
-  if (phase == 2 && dualcheck) launch_dualcheck(st_, m, i, it);
+  if (phase == 2 && dualcheck) {
+    if (force_due(DS_FORCE_DUAL_RESTART, it)) launch_force(st_, m, i, DS_FORCE_DUAL_RESTART);
+    launch_dualcheck(st_, m, i, it);
+  }
   launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
   prof_event(DS_PART_PRICING);
   launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
@@ -639,6 +712,7 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   // Bound flips (sequentialDualSimplex.cpp:398-422) exist only with boxed variables.
   const bool flips = phase == 2 && has_boxed_;
   if (flips) launch_bfrt(st_, m, i, f, P_.ratio_test, t);
+  if (phase == 2 && force_due(DS_FORCE_STABILITY, it)) launch_force(st_, m, i, DS_FORCE_STABILITY);
   prof_event(DS_PART_RATIO_TEST2);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
   prof_event(DS_PART_FTRAN);
@@ -663,8 +737,20 @@ void Solver::enqueue_stageB(int phase, int it, int t) {
 
 // Queue iterations next, next+1, ... up to the chunk size, stopping before
 // the next scheduled refactor and at the iteration cap.
+// Test hook: the forced event is due at iteration `it` (queued once; queued
+// again if the chunk holding it stopped before reaching it).
+bool Solver::force_due(int event, int it) {
+  if (P_.force_event != event || force_fired_ || force_enq_it_ >= 0 || it < P_.force_at) return false;
+  force_enq_it_ = it;
+  return true;
+}
+
 int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
   const int K = P_.refactor_freq;
+  if (force_enq_it_ >= 0) {
+    if (iteration_ >= force_enq_it_) force_fired_ = true;
+    force_enq_it_ = -1;
+  }
   int L = 0;
   for (int i = next;; ++i) {
     if (L > 0) {
@@ -679,6 +765,19 @@ int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
     if (P_.profile) break;  // one iteration per synchronisation: the step events are read back each time
   }
   return L;
+}
+
+// After a refactor that substituted slacks for dependent columns: the iterate
+// belongs to the old basis, so recompute it (exact DSE weights, then x and d,
+// or the phase-1 pricing vector).  Returns whether it did.
+bool Solver::after_repair(int phase) {
+  if (!repaired_) return false;
+  repaired_ = false;
+  restarts_++;
+  initBetaWeights(false);
+  if (phase == 2) dualSimplexInit();
+  else phase1Init();
+  return true;
 }
 
 // minimizeDualInfeasibility (baseDualSimplex.tpp:799-934).
@@ -711,6 +810,7 @@ int Solver::phase1() {
           phase1Init();
         }
         refactor();
+        after_repair(1);
         skip_first = 1;
       }
     }
@@ -732,7 +832,8 @@ int Solver::phase1() {
       hS_->refac_mid = 0;
       push();
       refactor();
-      enqueue_stageB(1, stop, 0);
+      // A repaired basis voids the pending basis change: start over from the new basis.
+      if (!after_repair(1)) enqueue_stageB(1, stop, 0);
       pull();
       t_ = hS_->t;
       continue;
@@ -765,6 +866,7 @@ int Solver::phase1() {
       case ST_PH1_RESTART: {
         restarts_++;
         refactor();
+        after_repair(1);
         phase1Init();
         initBetaWeights(false);
         pull();
@@ -792,6 +894,7 @@ int Solver::phase1() {
         hS_->any_blocked = 1;
         push();
         refactor();
+        after_repair(1);
         phase1Init();
         break;
       }
@@ -825,7 +928,10 @@ bool Solver::phase2() {
     } else {
       next = iteration_ + 1;
       tick_limit(next);
-      if (next % K == 0) refactor();
+      if (next % K == 0) {
+        refactor();
+        after_repair(2);
+      }
     }
     int L = plan_chunk(2, next, t_, skip_first);
     pull();
@@ -845,7 +951,8 @@ bool Solver::phase2() {
       hS_->refac_mid = 0;
       push();
       refactor();
-      enqueue_stageB(2, stop, 0);
+      // A repaired basis voids the pending basis change: start over from the new basis.
+      if (!after_repair(2)) enqueue_stageB(2, stop, 0);
       pull();
       t_ = hS_->t;
       continue;
@@ -864,6 +971,7 @@ bool Solver::phase2() {
         restarts_++;
         prof_host_begin();
         refactor();
+        after_repair(2);
         initBetaWeights(false);
         dualSimplexInit();
         prof_host_end(DS_PART_RESTORE);
@@ -872,6 +980,7 @@ bool Solver::phase2() {
         restarts_++;
         prof_host_begin();
         refactor();
+        after_repair(2);
         prof_host_end(DS_PART_RESTORE);
         pull();
         int cyc = hS_->cycle;
@@ -890,6 +999,7 @@ bool Solver::phase2() {
         prof_host_begin();
         perturbCosts();
         refactor();
+        after_repair(2);
         dualSimplexInit();
         prof_host_end(DS_PART_RESTORE);
         resume = stop;
@@ -908,6 +1018,7 @@ bool Solver::phase2() {
         push();
         prof_host_begin();
         refactor();
+        after_repair(2);
         initBetaWeights(false);
         prof_host_end(DS_PART_RESTORE);
         pull();
@@ -965,8 +1076,9 @@ int Solver::presolve() {
   t_start_ = t0;
   int st = 0;
   try {
+    repair_ = true;
     if (P_.perturb && checkPerturbNeed()) perturbCosts();
-    initBetaWeights(true);
+    initBetaWeights(!warm_);
     st = phase1();
     phase1_iters_ = iteration_;
     if (st == 1) {
@@ -1054,7 +1166,7 @@ void Solver::result(ds_result& r) const {
   r.nnz_nonbasic_mark = nnzN_mark_;
   r.nnz_nonbasic_end = nnzN_end_;
   r.cleanups = cleanups_;
-  r.pad0 = 0;
+  r.repairs = repairs_;
   for (int k = 0; k < DS_NPARTS; ++k) r.step_ms[k] = step_ms_[k];
   if (P_.profile) r.step_ms[DS_PART_FACTOR] = refactor_ms_;
 }
@@ -1082,8 +1194,27 @@ std::vector<int> Solver::nonbasis_host() {
   return b;
 }
 
+// initDualSimplex from a given starting basis: dependent columns give way to
+// slacks (the iterate is set up from the repaired basis), and presolve starts
+// from the exact DSE weights of that basis.
+void Solver::init_basis(const int* basis) {
+  initDualSimplex();
+  repairs_ = 0;
+  install_basis(basis);
+  repair_ = true;
+  refactor();
+  repaired_ = false;
+  warm_ = true;
+}
+
 void Solver::set_basis(const int* basis) {
   if (!inited_) initDualSimplex();
+  install_basis(basis);
+  repair_ = false;
+  refactor();
+}
+
+void Solver::install_basis(const int* basis) {
   const int m = H_.m, n = H_.n, N = H_.N;
   std::vector<int> b(basis, basis + m), nb, posN(N, -1);
   std::vector<char> isb(N, 0);
@@ -1101,7 +1232,6 @@ void Solver::set_basis(const int* basis) {
   upload(nonbasis_, nb, st_);
   upload(posN_, posN, st_);
   sync();
-  refactor();
 }
 
 void Solver::btran(int p_idx, double* rho) {

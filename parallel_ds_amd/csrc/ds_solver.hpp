@@ -82,6 +82,8 @@ class Solver {
 
   // hook-level entries
   void set_basis(const int* basis);
+  void init_basis(const int* basis);
+  void install_basis(const int* basis);
   void refactor();
   void btran(int p_idx, double* rho);
   void ftran(const double* b, double* x);
@@ -126,6 +128,19 @@ class Solver {
       rowsB_, nA_, nB_;
   int k_ = 0;
   int eta_cap_ = 0;
+  // slack substitution for dependent kernel columns
+  bool repair_ = false;    // substitute (warm starts, refactors inside a solve) or fail
+  bool repaired_ = false;  // the last refactor changed the basis
+  bool warm_ = false;      // the solve starts from a given basis (exact DSE weights)
+  // test hook (params.force_event)
+  bool force_fired_ = false;
+  int force_enq_it_ = -1;
+  bool force_due(int event, int it);
+  int repairs_ = 0;
+  std::vector<int> fac_pos_P_, fac_row_R_;
+  void factor_once(std::vector<std::pair<int, int>>& dep);
+  void substitute_slacks(const std::vector<std::pair<int, int>>& dep);
+  bool after_repair(int phase);
   int gj_mfma_ = 1;  // trailing update of the inversion on FP64 MFMA
 
   // host mirrors of the control state

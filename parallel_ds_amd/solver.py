@@ -70,8 +70,16 @@ class DualSimplex:
         self.close()
 
     # reference call sequence (src/main.cpp:57-68)
-    def initDualSimplex(self):
-        self._check(lib().ds_init(self._ctx), "ds_init")
+    def initDualSimplex(self, start=None):
+        """From the slack basis, or from `start` (m variable ids), whose dependent columns the
+        first factorization replaces by slacks (Result.repairs)."""
+        if start is None:
+            self._check(lib().ds_init(self._ctx), "ds_init")
+        else:
+            b = np.ascontiguousarray(start, np.int32)
+            if b.shape != (self.lp.m,):
+                raise ValueError("a starting basis lists m variables")
+            self._check(lib().ds_init_basis(self._ctx, ptr(b)), "ds_init_basis")
 
     def presolve(self, method="minInfeas"):
         st = C.c_int()
@@ -153,10 +161,10 @@ class DualSimplex:
         return w.value, c.value, by.value
 
 
-def solve_lp(lp, params=None, device=0):
+def solve_lp(lp, params=None, device=0, start=None):
     """load + initDualSimplex + presolve("minInfeas") + solve; returns (LPsolution, basis, Result)."""
     with DualSimplex(lp, params, device) as s:
-        s.initDualSimplex()
+        s.initDualSimplex(start)
         st = s.presolve("minInfeas")
         if st == 0:
             s.solve()
