@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X dual simplex hot path.
 
-Workload (default): the single large sparse LP, FP64, m=20,000 rows, n=40,000
-columns, exactly 10 nonzeros per column at uniform-random rows, values U(0.1,1),
-b = A x0 + U(0,1), c = -U(0,1), refactor every 64 iterations with PFI etas in
-between (seed 1).  A step is one dual simplex iteration (pricing, BTRAN,
-pivot-row SpMV, ratio test, FTRAN, updates, periodic refactor) from the slack
-basis, phase 1 and phase 2 on the shared iteration counter.  --warmup W
-iterations run untimed, then exactly --steps K iterations are timed between
-two device synchronisations; inputs are resident in HBM before the timed region.
+Headline (the metric: solves/s, learned-warm vs slack-start on the same GPU and the same LPs):
+the LP family of the single-LP recipe at m=2,000, n=5,000 with 5 nonzeros per column (base
+seed 4), b and c times lognormal(0, 0.1), FP64.  The per-variable network (BF16 MFMA kernels) is
+trained on the optimal bases the GPU solver finds for --train instances; a step is one held-out
+instance solved on the single-LP GPU path (--threads at a time, one HIP stream each).  --warmup W
+held-out instances run untimed, then --steps K instances are solved from the slack basis and from
+the learned warm start; `value` is the learned-warm solves/s over those K, with the slack-start
+solves/s, iterations and objective agreement beside it.  Inputs are resident in HBM before each
+solve starts; the wall time includes the host's per-chunk synchronisation.
 
-Prints one JSON line (rank 0) with the roofline of the pivot-row SpMV measured
-with HIP events on the solver's stream over the timed iterations, and the CPU
-baseline: the oracle (a C++ restatement of the reference's sequential solver)
-timed on the same instance over the same iteration window, single thread.
+Also reported: the single large LP (m=20,000, n=40,000, 10 nonzeros per column, refactor every
+64) over a fixed window of iterations 501..2,500 with the pivot-row SpMV roofline (`roofline`,
+HIP events on the solver's stream), the FP64 MFMA use of the refactorization's trailing update,
+the batched small-LP family, the SpMV size sweep, the transportation LP with the Harris ratio test,
+and the batched learned warm start at m=256.  CPU baselines: the oracle (a C++ restatement of the
+reference's sequential solver), single thread unless stated.
 """
 import argparse
 import json
@@ -367,17 +370,225 @@ def learned_warm(n_train, n_eval, epochs, reps, cpu_sample, plain_sample=20, m=2
     return out
 
 
+def _solve_many(items, prm_of, threads):
+    """Solve LPs on the single-LP path, `threads` contexts (streams) at a time; items are
+    (lp, start basis or None). Returns (wall s, per-LP (s, status, Z, iterations, repairs, basis,
+    solve_ms, phase1 iterations, restarts))."""
+    import concurrent.futures as cf
+    from parallel_ds_amd import solve_lp
+
+    def run(item):
+        lp, start, prm = item
+        t = time.perf_counter()
+        sol, basis, r = solve_lp(lp, prm, start=start)
+        return (time.perf_counter() - t, r.status, sol.Z, r.iterations, r.repairs, basis, r.solve_ms,
+                r.phase1_iterations, r.restarts)
+
+    t0 = time.perf_counter()
+    out = [None] * len(items)
+    with cf.ThreadPoolExecutor(threads) as ex:
+        futs = {ex.submit(run, (lp, st, prm_of(k))): k for k, (lp, st) in enumerate(items)}
+        for f in cf.as_completed(futs):
+            out[futs[f]] = f.result()
+    return time.perf_counter() - t0, out
+
+
+def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_iters, cpu_from=1500, m=2000,
+                    n=5000, k=5, seed=4, sigma=0.1):
+    """Learned warm start against slack start at the family's stated size: the single-LP recipe at
+    m=2,000, n=5,000 with 5 nonzeros per column (base seed 4), b and c times lognormal(0, 0.1).
+    The network is trained on the optimal bases the GPU solver finds for n_train instances (seeds
+    10000+); `warmup` held-out instances (seeds 20000+) run untimed, then `steps` held-out instances
+    (the next seeds) are solved from the slack basis and from the learned warm start, `threads` LPs
+    at a time (one context and HIP stream each). A warm solve that has not ended optimal within
+    twice the mean training solve time is re-solved from the slack basis (the fallback the slack
+    start provides); its time counts in full."""
+    import numpy as np
+    from parallel_ds_amd import default_params
+    from parallel_ds_amd import lp as lpmod
+    from parallel_ds_amd.batch import BatchSolver
+    from parallel_ds_amd.warm import Mlp
+    base = lpmod.gen_kcol(m, n, k, seed)
+    train = [lpmod.perturbed_instance(base, sigma, 10000 + s) for s in range(n_train)]
+    held = [lpmod.perturbed_instance(base, sigma, 20000 + s) for s in range(warmup + steps)]
+    slack_prm = default_params(refactor_freq=refactor_freq)
+    log("learned warm start m=%d n=%d: solving %d training instances, %d at a time" % (m, n, n_train, threads))
+    t_tr, tr = _solve_many([(lp, None) for lp in train], lambda i: slack_prm, threads)
+    bases = [r[5] for r in tr]
+    limit_ms = 2.0 * float(np.mean([r[6] for r in tr]))
+    warm_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=limit_ms)
+    with BatchSolver(train) as T, Mlp(seed=1) as net:
+        T.set_bases(bases)
+        losses, fit_ms = net.fit(T, epochs=epochs, minibatch=8192, lr=1e-3, seed=3)
+        log("learned warm start: losses %s" % [round(float(v), 4) for v in losses])
+        tr_ms, tr_flops = net.train_stats()
+        with BatchSolver(held) as E:
+            t0 = time.perf_counter()
+            E.warm_start(net)
+            pred_s = time.perf_counter() - t0
+            warm = E.get_warm()
+    overlap = []
+
+    def warm_phase(lps, starts):
+        t0 = time.perf_counter()
+        wall_w, rw = _solve_many(list(zip(lps, starts)), lambda i: warm_prm, threads)
+        redo = [i for i, r in enumerate(rw) if r[1] != 0]
+        if redo:
+            _, rf = _solve_many([(lps[i], None) for i in redo], lambda i: slack_prm, threads)
+            for i, r in zip(redo, rf):
+                rw[i] = r
+        return time.perf_counter() - t0, rw, len(redo)
+
+    if warmup > 0:
+        log("learned warm start: %d untimed held-out instances" % warmup)
+        _solve_many([(lp, None) for lp in held[:warmup]], lambda i: slack_prm, threads)
+        warm_phase(held[:warmup], warm[:warmup])
+    ev, ew = held[warmup:], warm[warmup:]
+    log("learned warm start: %d held-out instances from the slack basis" % steps)
+    wall_s, rs = _solve_many([(lp, None) for lp in ev], lambda i: slack_prm, threads)
+    log("learned warm start: the same %d from the learned warm start" % steps)
+    wall_w, rw, fallbacks = warm_phase(ev, ew)
+    wall_w += pred_s * steps / len(held)
+    overlap = [len(set(w.tolist()) & set(r[5].tolist())) / m for w, r in zip(ew, rs)]
+    ok = all(a[1] == 0 and b[1] == 0 for a, b in zip(rs, rw))
+    rel = max(abs(a[2] - b[2]) / max(1.0, abs(a[2])) for a, b in zip(rs, rw))
+    out = {
+        "workload": "LP family m=%d n=%d, %d nnz/col (base seed %d), b and c x lognormal(0, %g); trained on %d "
+                    "GPU-solved instances (seeds 10000+), %d untimed + %d timed held-out instances (seeds 20000+), "
+                    "%d solves at a time, refactor every %d" % (m, n, k, seed, sigma, n_train, warmup, steps, threads,
+                                                               refactor_freq),
+        "slack": {"solves_per_s": round(steps / wall_s, 4), "wall_s": round(wall_s, 3),
+                  "mean_iterations": round(float(np.mean([r[3] for r in rs])), 1),
+                  "mean_phase1_iterations": round(float(np.mean([r[7] for r in rs])), 1),
+                  "mean_solve_ms": round(float(np.mean([r[6] for r in rs])), 1),
+                  "ms_per_iter": round(float(np.sum([r[6] for r in rs]) / np.sum([r[3] for r in rs])), 4)},
+        "learned_warm": {"solves_per_s": round(steps / wall_w, 4), "wall_s": round(wall_w, 3),
+                         "mean_iterations": round(float(np.mean([r[3] for r in rw])), 1),
+                         "mean_phase1_iterations": round(float(np.mean([r[7] for r in rw])), 1),
+                         "mean_restarts": round(float(np.mean([r[8] for r in rw])), 1),
+                         "fallbacks_to_slack": fallbacks, "time_limit_ms": round(limit_ms, 1),
+                         "repaired_columns_mean": round(float(np.mean([r[4] for r in rw])), 2),
+                         "predict_s": round(pred_s, 4)},
+        "speedup_solves_per_s": round(wall_s / wall_w, 4),
+        "predicted_basis_overlap_with_optimal": round(float(np.mean(overlap)), 4),
+        "all_optimal": bool(ok), "max_obj_rel_err_warm_vs_slack": float(rel),
+        "training": {"instances": n_train, "solve_wall_s": round(t_tr, 2),
+                     "mean_iterations": round(float(np.mean([r[3] for r in tr])), 1),
+                     "epochs": epochs, "epoch_loss": [round(float(v), 4) for v in losses], "fit_ms": round(fit_ms, 1),
+                     "train_gemm_tflops": round(tr_flops / (tr_ms * 1e-3) / 1e12, 2) if tr_ms > 0 else None},
+    }
+    if cpu_iters > 0:
+        # The oracle on the first timed instance: a bounded window from the slack basis and from the
+        # same learned start, at the same refactor interval, 1 thread.
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as ora  # test-side binding of the oracle
+        log("learned warm start: CPU oracle, iterations %d..%d from the slack basis" % (cpu_from + 1, cpu_from + cpu_iters))
+        t0 = time.perf_counter()
+        ora.solve(ev[0], refactor_freq=refactor_freq, max_iterations=cpu_from, tie_rule=1)
+        t1 = time.perf_counter()
+        ro, _, _ = ora.solve(ev[0], refactor_freq=refactor_freq, max_iterations=cpu_from + cpu_iters, tie_rule=1)
+        s_slack = (time.perf_counter() - t1) - (t1 - t0)
+        done = ro.iterations - cpu_from
+        t0 = time.perf_counter()
+        try:
+            rw0, _, _ = ora.solve_from(ev[0], ew[0], refactor_freq=refactor_freq, max_iterations=cpu_iters,
+                                       tie_rule=1)
+            warm_ips = round(rw0.iterations / (time.perf_counter() - t0), 2)
+        except RuntimeError:
+            warm_ips = None  # the predicted basis is singular; the oracle has no slack substitution
+        ips = done / s_slack
+        out["cpu_baseline"] = {
+            "value": round(ips / out["slack"]["mean_iterations"], 6), "unit": "solves/s", "cores": 1, "kind": "port",
+            "iter_per_s_slack": round(ips, 2), "iter_per_s_warm": warm_ips,
+            "sample": "oracle on held-out instance %d: iterations %d..%d from the slack basis (%.1f s: a %d-iteration "
+                      "run minus a %d-iteration run) and %d from the learned start (when it factors without "
+                      "substitution), 1 thread, %s; solves/s = that window's iterations/s / the GPU's mean slack "
+                      "iterations per solve (later iterations cost the oracle more: an upper bound)"
+                      % (warmup, cpu_from + 1, cpu_from + done, s_slack, cpu_from + cpu_iters, cpu_from, cpu_iters,
+                         cpu_model())}
+    return out
+
+
+def single_lp_window(args):
+    """The single large LP (m=20,000, n=40,000, 10 nonzeros per column, refactor every 64) over a
+    fixed iteration window from the slack basis (iterations c1_from+1 .. c1_from+c1_window, default
+    501..2,500), its pivot-row SpMV roofline from event pairs on every 8th SpMV launch of the window,
+    the kernel size and refactor time, and the oracle over the same window."""
+    from parallel_ds_amd import DualSimplex, default_params
+    from parallel_ds_amd import lp as lpmod
+    lp = lpmod.config1(seed=args.seed, m=args.m, n=args.n, k=args.k)
+    w0, w = args.c1_from, args.c1_window
+    log("single LP m=%d n=%d: iterations %d..%d" % (lp.m, lp.n, w0 + 1, w0 + w))
+    prm = default_params(refactor_freq=args.refactor, chunk=args.chunk, max_iterations=w0 + w,
+                         mark_iteration=w0, time_kernels=SPMV_TIME_STRIDE)
+    with DualSimplex(lp, prm) as s:
+        s.initDualSimplex()
+        if s.presolve("minInfeas") == 0:
+            s.solve()
+        r = s.result()
+    timed = (r.iterations - w0) if (r.iterations < w0 + w and r.status != 3) else w
+    window_ms = r.solve_ms - r.mark_ms
+    spmv_avg_ms = r.spmv_ms / max(r.spmv_launches, 1)
+    nnzN = 0.5 * (r.nnz_nonbasic_mark + r.nnz_nonbasic_end)
+    spmv_bytes = 12.0 * nnzN + 4.0 * (lp.n + 1) + 8.0 * lp.n + 8.0 * lp.m
+    achieved = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
+    out = {"workload": "single sparse LP m=%d n=%d, %d nnz/col, seed %d, refactor every %d, iterations %d..%d from "
+                       "the slack basis" % (lp.m, lp.n, args.k, args.seed, args.refactor, w0 + 1, w0 + timed),
+           "iter_per_s": round(timed / (window_ms * 1e-3), 2), "ms_per_iter": round(window_ms / max(timed, 1), 5),
+           "status_at_end": r.status, "kernel_size_at_end": r.kernel_k, "refactors": r.refactors,
+           "refactor_ms_total": round(r.refactor_ms, 2),
+           "roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": SPMV_TRAFFIC_BYTES.get((args.m, args.n, args.k, args.seed, args.refactor)),
+                        "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
+                        "launches": r.spmv_launches}}
+    if not args.no_cpu_baseline:
+        log("single LP: CPU oracle over the same window")
+        out["cpu_baseline"] = cpu_baseline(lp, args.refactor, w0, timed)
+    return out
+
+
+def refactor_mfma(k_target=1600, reps=3):
+    """FP64 MFMA use of the refactorization's trailing update (k_gemm_sub) on a basis of the
+    learned-warm-start family's size: the slack solve's final basis (kernel k ~ 1,600), factored
+    `reps` times with an event pair around every trailing-update launch. Executed flops per launch:
+    2 (k - nb) k nb for a panel of nb columns."""
+    from parallel_ds_amd import DualSimplex, default_params, solve_lp
+    from parallel_ds_amd import lp as lpmod
+    lp = lpmod.gen_kcol(2000, 5000, 5, 4)
+    sol, basis, r = solve_lp(lp, default_params(refactor_freq=150))
+    with DualSimplex(lp, default_params()) as s:
+        s.initDualSimplex()
+        s.set_basis(basis)
+        t = s.time_refactor(reps)
+    ach = t["gemm_flops"] / (t["gemm_ms"] * 1e-3) / 1e12 if t["gemm_ms"] > 0 else 0.0
+    tot = 2.0 * t["k"] ** 3 / (t["refactor_ms"] * 1e-3) / 1e12 if t["refactor_ms"] > 0 else 0.0
+    return {"kernel": "k_gemm_sub (FP64 MFMA trailing update)", "bound": "mfma", "k": t["k"],
+            "achieved": round(ach, 3), "peak": 78.6, "unit": "TFLOP/s", "frac": round(ach / 78.6, 4),
+            "gemm_ms_per_refactor": round(t["gemm_ms"], 3), "refactor_ms": round(t["refactor_ms"], 3),
+            "refactor_tflops_2k3": round(tot, 3), "refactor_frac_2k3": round(tot / 78.6, 4),
+            "sample": "basis of the optimal slack solve of the family base LP (m=2000, n=5000), %d factorizations"
+                      % reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=8, help="held-out family LPs timed (per start)")
+    ap.add_argument("--warmup", type=int, default=1, help="held-out family LPs solved untimed first")
+    ap.add_argument("--train", type=int, default=16, help="GPU-solved training instances of the family")
+    ap.add_argument("--threads", type=int, default=4, help="family LPs solved at a time (contexts/streams)")
+    ap.add_argument("--family-refactor", type=int, default=150)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--family-cpu-iters", type=int, default=300)
     ap.add_argument("--m", type=int, default=20000)
     ap.add_argument("--n", type=int, default=40000)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--refactor", type=int, default=64)
     ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--c1-from", type=int, default=500)
+    ap.add_argument("--c1-window", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -390,6 +601,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-transport", action="store_true")
     ap.add_argument("--no-spmv-sweep", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size once (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -400,72 +613,46 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
-    from parallel_ds_amd import DualSimplex, default_params
-    from parallel_ds_amd import lp as lpmod
-
-    lp = lpmod.config1(seed=args.seed, m=args.m, n=args.n, k=args.k)
-    log("single LP m=%d n=%d: %d warmup + %d timed iterations" % (lp.m, lp.n, args.warmup, args.steps))
-    prm = default_params(refactor_freq=args.refactor, chunk=args.chunk,
-                         max_iterations=args.warmup + args.steps,
-                         mark_iteration=args.warmup, time_kernels=SPMV_TIME_STRIDE)
-    with DualSimplex(lp, prm) as s:
-        s.initDualSimplex()
-        st = s.presolve("minInfeas")
-        if st == 0:
-            s.solve()
-        r = s.result()
-    if r.iterations < args.warmup + args.steps and r.status != 3:
-        # solved before the window ended: time what ran after the mark
-        timed = r.iterations - args.warmup
-    else:
-        timed = args.steps
-    window_ms = r.solve_ms - r.mark_ms
-    ms_per_iter = window_ms / max(timed, 1)
-    spmv_avg_ms = r.spmv_ms / max(r.spmv_launches, 1)
-    n_nonbasic = lp.n
-    nnzN = 0.5 * (r.nnz_nonbasic_mark + r.nnz_nonbasic_end)
-    spmv_bytes = 12.0 * nnzN + 4.0 * (n_nonbasic + 1) + 8.0 * n_nonbasic + 8.0 * lp.m
-    achieved = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
-
+    fam = learned_warm_c4(args.steps, args.warmup, args.train, args.threads, args.family_refactor, args.epochs,
+                          0 if args.no_cpu_baseline else args.family_cpu_iters)
+    single = None if args.no_single else single_lp_window(args)
     out = {
         "metric": METRIC,
-        "value": round(timed / (window_ms * 1e-3), 3),
-        "unit": "iter/s",
+        "value": fam["learned_warm"]["solves_per_s"],
+        "unit": "solves/s",
         "n_gpus": 1,
-        "steps": timed,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_iter, 5),
+        "ms_per_step": round(1e3 * fam["learned_warm"]["wall_s"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 LP generator)",
-        "config": {"workload": "single sparse LP m=%d n=%d, %d nnz/col, refactor every %d, iterations %d..%d "
-                               "from the slack basis" % (lp.m, lp.n, args.k, args.refactor, args.warmup + 1,
-                                                         args.warmup + timed),
-                   "m": lp.m, "n": lp.n, "nnz": lp.nnz, "seed": args.seed, "refactor_freq": args.refactor,
-                   "chunk": args.chunk, "status_at_end": r.status, "kernel_size_at_end": r.kernel_k,
-                   "refactors": r.refactors, "refactor_ms_total": round(r.refactor_ms, 2)},
-        "roofline": {"kernel": "k_spmv (pivot-row SpMV)", "bound": "hbm",
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": SPMV_TRAFFIC_BYTES.get((args.m, args.n, args.k, args.seed, args.refactor)),
-                     "bytes_per_launch": round(spmv_bytes), "avg_launch_us": round(spmv_avg_ms * 1e3, 3),
-                     "launches": r.spmv_launches},
+        "config": {"workload": fam["workload"], "m": 2000, "n": 5000, "nnz_per_col": 5, "sigma": 0.1,
+                   "train_instances": args.train, "held_out_timed": args.steps, "held_out_untimed": args.warmup,
+                   "concurrent_solves": args.threads, "refactor_freq": args.family_refactor,
+                   "slack_solves_per_s": fam["slack"]["solves_per_s"], "speedup": fam["speedup_solves_per_s"]},
+        "roofline": single["roofline"] if single else None,
+        "cpu_baseline": fam.get("cpu_baseline"),
+        "learned_warm_family_m2000": fam,
     }
-    if not args.no_cpu_baseline:
-        log("single LP: CPU oracle over the same window")
-        out["cpu_baseline"] = cpu_baseline(lp, args.refactor, args.warmup, timed)
+    if single:
+        out["single_lp"] = single
+    if not args.no_mfma:
+        log("refactorization: FP64 MFMA timing")
+        out["refactor_mfma"] = refactor_mfma()
     if not args.no_batch:
         out["batched"] = batched_family(args.batch_count, 256, args.batch_reps,
                                         0 if args.no_cpu_baseline else args.batch_cpu_sample)
     if not args.no_spmv_sweep:
         out["spmv_sweep"] = spmv_sweep()
     if not args.no_transport:
-        out["transport"] = transport(not args.no_cpu_baseline)
+        out["transport"] = transport(not args.no_cpu_baseline, ratio_test=1)
     if not args.no_warm:
-        out["learned_warm_start"] = learned_warm(args.warm_train, args.warm_eval, args.warm_epochs, args.batch_reps,
-                                                 0 if args.no_cpu_baseline else args.warm_cpu_sample)
+        out["learned_warm_start_m256"] = learned_warm(args.warm_train, args.warm_eval, args.warm_epochs,
+                                                      args.batch_reps,
+                                                      0 if args.no_cpu_baseline else args.warm_cpu_sample)
     if args.copy_calibration_mib > 0:
         out["copy_calibration"] = copy_calibration(args.copy_calibration_mib)
     print(json.dumps(out), flush=True)

@@ -45,7 +45,7 @@ enum {
   DS_OPTIMAL = 0,            /* "optimal solution" */
   DS_PRIMAL_INFEASIBLE = 1,  /* "dual unbounded" */
   DS_DUAL_INFEASIBLE = 2,    /* "dual infeasible" */
-  DS_ITERATION_LIMIT = 3,
+  DS_ITERATION_LIMIT = 3,    /* max_iterations or time_limit_ms reached */
   DS_NOT_SOLVED = 4
 };
 
@@ -79,6 +79,8 @@ typedef struct {
   int force_event;        /* test hook, 0 = off: force one recovery path at iteration force_at:
                              DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART or DS_FORCE_SINGULAR */
   int force_at;
+  double time_limit_ms;   /* > 0: end a solve that has run this long (phase 1 + phase 2 wall time) with
+                             DS_ITERATION_LIMIT and the message "time limit"; 0 = none (reference) */
 } ds_params;
 
 /* Recovery paths a test can force from the host (sequentialDualSimplex.cpp):
@@ -303,6 +305,13 @@ int ds_batch_get_warm(ds_ctx* ctx, int* basis /* sum m */);
  * substitutes a slack (default 1e-7; 0 disables: singular bases fail). */
 int ds_batch_set_repair_tol(ds_ctx* ctx, double tol);
 int ds_batch_repairs(ds_ctx* ctx, int* count /* per LP */);
+
+/* Factorizations of the context's current basis, `reps` times (measurement
+ * entry, no counterpart in the reference): *refactor_ms = mean wall time of one
+ * refactorization; *gemm_ms = device time of its FP64 MFMA trailing updates
+ * (event pairs per panel); *gemm_flops = their executed flops, 2 (k - nb) k nb
+ * per panel of nb columns; *k = kernel size. */
+int ds_time_refactor(ds_ctx* ctx, int reps, double* refactor_ms, double* gemm_ms, double* gemm_flops, int* k);
 
 /* Counter calibration: one plain copy kernel over `bytes` (a multiple of 16)
  * between two fresh device buffers; *ms = its duration. */

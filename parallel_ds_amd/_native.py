@@ -19,7 +19,8 @@ class Params(C.Structure):
     _fields_ = [("refactor_freq", C.c_int), ("max_iterations", C.c_int), ("ratio_test", C.c_int),
                 ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
                 ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
-                ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int)]
+                ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int),
+                ("time_limit_ms", C.c_double)]
 
 
 DS_FORCE_NONE, DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART, DS_FORCE_SINGULAR = 0, 1, 2, 3
@@ -86,6 +87,7 @@ _SIGS = {
     "ds_time_pivot_row": (I32, [VP, VP, I32, C.POINTER(F64), C.POINTER(F64)]),
     "ds_time_pivot_row_ext": (I32, [VP, VP, I32, C.c_longlong, C.POINTER(F64), C.POINTER(F64), C.POINTER(F64)]),
     "ds_debug_gemm_tile": (I32, [VP, VP, VP, I32, I32]),
+    "ds_time_refactor": (I32, [VP, I32, C.POINTER(F64), C.POINTER(F64), C.POINTER(F64), C.POINTER(I32)]),
     "ds_debug_copy": (I32, [C.c_longlong, C.POINTER(F64)]),
     "ds_batch_default_params": (None, [C.POINTER(BatchParams)]),
     "ds_batch_load": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP]),

@@ -102,6 +102,7 @@ void ds_default_params(ds_params* p) {
   p->profile = 0;
   p->force_event = DS_FORCE_NONE;
   p->force_at = 0;
+  p->time_limit_ms = 0.0;
 }
 
 int ds_create(int device, ds_ctx** out) {
@@ -169,6 +170,11 @@ int ds_set_params(ds_ctx* ctx, const ds_params* p) {
 
 int ds_init(ds_ctx* ctx) {
   return guard(ctx, [&](pds::Solver& s) { s.initDualSimplex(); });
+}
+
+int ds_time_refactor(ds_ctx* ctx, int reps, double* refactor_ms, double* gemm_ms, double* gemm_flops, int* k) {
+  if (reps <= 0 || !refactor_ms || !gemm_ms || !gemm_flops || !k) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) { s.time_refactor(reps, *refactor_ms, *gemm_ms, *gemm_flops, *k); });
 }
 
 int ds_init_basis(ds_ctx* ctx, const int* basis) {

@@ -56,6 +56,7 @@ struct GJWork {
   int* nB;
   int use_mfma;   // trailing update on FP64 MFMA (1) or plain FMA (0)
   double* rowscale;  // k: largest |entry| of each row of K, kept in step with the row swaps
+  hipEvent_t* gemm_ev = nullptr;  // optional: start/stop event pair around each panel's trailing update
 };
 constexpr int GJ_NB = 64;
 constexpr int GJ_DEP_CAP = 4096;  // dependent kernel columns one factorization can report

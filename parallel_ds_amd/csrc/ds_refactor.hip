@@ -11,6 +11,8 @@
 // are written.  Total work 2k^3 flops, all but O(k^2 NB) in the MFMA update.
 #include "ds_dev.hpp"
 #include "ds_kernels.hpp"
+
+#include <hip/hip_ext.h>
 #include "ds_reduce.hpp"
 
 namespace pds {
@@ -547,7 +549,10 @@ void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
     k_panel_save<<<(int)(((long)k * nb + 255) / 256), 256, 0, s>>>(G.W, k, j, nb, G.Pb);
     k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv);
     dim3 grid((k + 63) / 64, (k + 63) / 64);
-    if (G.use_mfma)
+    if (G.use_mfma && G.gemm_ev)
+      hipExtLaunchKernelGGL(k_gemm_sub, grid, dim3(256), 0, s, G.gemm_ev[2 * (j / NB)], G.gemm_ev[2 * (j / NB) + 1], 0,
+                            G.W, k, k, k, (const double*)G.Pb, (const double*)(G.W + (size_t)j * k), k, nb, j, j + nb);
+    else if (G.use_mfma)
       k_gemm_sub<<<grid, 256, 0, s>>>(G.W, k, k, k, G.Pb, G.W + (size_t)j * k, k, nb, j, j + nb);
     else
       k_gemm_sub_fma<<<grid, 256, 0, s>>>(G.W, k, k, k, G.Pb, G.W + (size_t)j * k, k, nb, j, j + nb);

@@ -285,6 +285,10 @@ void Solver::tick_limit(int it) {
     iteration_ = P_.max_iterations;
     throw IterLimit{};
   }
+  if (P_.time_limit_ms > 0.0 && now_ms() - t_start_ > P_.time_limit_ms) {
+    time_hit_ = true;
+    throw IterLimit{};
+  }
 }
 
 template <typename T>
@@ -516,6 +520,7 @@ void Solver::factor_once(std::vector<std::pair<int, int>>& dep) {
       force_fired_ = true;
     }
     GJWork G{GW_.p, Pb_.p, piv_.p, perm_.p, flag_.p, Pinv_.p, rowsA_.p, rowsB_.p, nA_.p, nB_.p, gj_mfma_, rowscale_.p};
+    if ((int)gemm_ev_.size() >= 2 * ((k + GJ_NB - 1) / GJ_NB)) G.gemm_ev = gemm_ev_.data();
     launch_gj_panels(st_, k, G);
     int flag[2] = {0, 0};
     std::vector<int> piv(k), perm(k);
@@ -672,6 +677,7 @@ void Solver::initDualSimplex() {
   repair_ = false;
   force_fired_ = false;
   force_enq_it_ = -1;
+  time_hit_ = false;
   for (double& v : step_ms_) v = 0.0;
   pev_part_.clear();
   solve_ms_ = refactor_ms_ = 0.0;
@@ -1091,7 +1097,7 @@ int Solver::presolve() {
     sync();
   } catch (const IterLimit&) {
     status_ = DS_ITERATION_LIMIT;
-    message_ = "iteration limit";
+    message_ = time_hit_ ? "time limit" : "iteration limit";
     st = 2;
     solve_ms_ += now_ms() - t0;
     if (P_.time_kernels) nnzN_end_ = nnz_nonbasic();
@@ -1140,7 +1146,7 @@ void Solver::solve() {
     Z_ = hS_->obj;
   } catch (const IterLimit&) {
     status_ = DS_ITERATION_LIMIT;
-    message_ = "iteration limit";
+    message_ = time_hit_ ? "time limit" : "iteration limit";
     pull();
     Z_ = hS_->obj;
   }
@@ -1302,6 +1308,43 @@ void Solver::time_pivot_row_ext(const double* rho, int reps, long long flush_byt
   double nnzN = 0;
   for (int j : nb) nnzN += H_.cptr[j + 1] - H_.cptr[j];
   bytes = 12.0 * nnzN + 4.0 * (nb.size() + 1) + 8.0 * nb.size() + 8.0 * H_.m;
+}
+
+// Factorizations of the current basis, `reps` times: mean wall time of a
+// refactorization, and the device time and executed flops (2 (k - nb) k nb per
+// panel of nb columns) of its FP64 MFMA trailing updates from event pairs.
+void Solver::time_refactor(int reps, double& refactor_ms, double& gemm_ms, double& gemm_flops, int& k) {
+  refactor();
+  k = k_;
+  const int panels = (k_ + GJ_NB - 1) / GJ_NB;
+  for (int i = (int)gemm_ev_.size(); i < 2 * panels; ++i) {
+    hipEvent_t e;
+    PDS_HIP(hipEventCreate(&e));
+    gemm_ev_.push_back(e);
+  }
+  double tr = 0.0, tg = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    sync();
+    const double t0 = now_ms();
+    refactor();
+    sync();
+    tr += now_ms() - t0;
+    for (int p = 0; p < panels; ++p) {
+      float ms = 0.f;
+      PDS_HIP(hipEventElapsedTime(&ms, gemm_ev_[2 * p], gemm_ev_[2 * p + 1]));
+      tg += ms;
+    }
+  }
+  for (hipEvent_t e : gemm_ev_) (void)hipEventDestroy(e);
+  gemm_ev_.clear();
+  double fl = 0.0;
+  for (int j = 0; j < k_; j += GJ_NB) {
+    const double nb = std::min(GJ_NB, k_ - j);
+    fl += 2.0 * (k_ - nb) * (double)k_ * nb;
+  }
+  refactor_ms = reps > 0 ? tr / reps : 0.0;
+  gemm_ms = reps > 0 ? tg / reps : 0.0;
+  gemm_flops = fl;
 }
 
 void Solver::time_pivot_row(const double* rho, int reps, double& ms, double& bytes) {

@@ -90,6 +90,7 @@ class Solver {
   void ftran_column(int j, double* x);
   void pivot_row(const double* rho, double* alpha);
   void time_pivot_row(const double* rho, int reps, double& ms, double& bytes);
+  void time_refactor(int reps, double& refactor_ms, double& gemm_ms, double& gemm_flops, int& k);
   void time_pivot_row_ext(const double* rho, int reps, long long flush_bytes, double& warm_ms, double& cold_ms,
                           double& bytes);
 
@@ -136,6 +137,8 @@ class Solver {
   bool force_fired_ = false;
   int force_enq_it_ = -1;
   bool force_due(int event, int it);
+  bool time_hit_ = false;  // the wall-clock limit ended the solve
+  std::vector<hipEvent_t> gemm_ev_;  // time_refactor: event pairs around the trailing updates
   int repairs_ = 0;
   std::vector<int> fac_pos_P_, fac_row_R_;
   void factor_once(std::vector<std::pair<int, int>>& dep);

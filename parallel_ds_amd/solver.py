@@ -152,6 +152,13 @@ class DualSimplex:
                     "ds_time_pivot_row")
         return ms.value, by.value
 
+    def time_refactor(self, reps=3):
+        """Factorizations of the current basis: mean ms, trailing-update (FP64 MFMA) ms and flops, k."""
+        r, g, f, k = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        self._check(lib().ds_time_refactor(self._ctx, reps, C.byref(r), C.byref(g), C.byref(f), C.byref(k)),
+                    "ds_time_refactor")
+        return {"refactor_ms": r.value, "gemm_ms": g.value, "gemm_flops": f.value, "k": k.value}
+
     def time_pivot_row_ext(self, rho, reps=100, flush_bytes=640 << 20):
         """(warm ms, cold ms, algorithmic bytes) per SpMV launch; cold flushes caches before each launch."""
         rho = np.ascontiguousarray(rho, np.float64)
