@@ -389,12 +389,14 @@ def _solve_many(items, prm_of, threads):
     with cf.ThreadPoolExecutor(threads) as ex:
         futs = {ex.submit(run, (lp, st, prm_of(k))): k for k, (lp, st) in enumerate(items)}
         for f in cf.as_completed(futs):
-            out[futs[f]] = f.result()
+            k = futs[f]
+            out[k] = f.result()
+            log("  %s: status %d, %d iterations, %.2f s" % (items[k][0].name, out[k][1], out[k][3], out[k][0]))
     return time.perf_counter() - t0, out
 
 
 def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_iters, cpu_from=1500, m=2000,
-                    n=5000, k=5, seed=4, sigma=0.1):
+                    n=5000, k=5, seed=4, sigma=0.1, slack_limit_s=60.0):
     """Learned warm start against slack start at the family's stated size: the single-LP recipe at
     m=2,000, n=5,000 with 5 nonzeros per column (base seed 4), b and c times lognormal(0, 0.1).
     The network is trained on the optimal bases the GPU solver finds for n_train instances (seeds
@@ -411,11 +413,16 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
     base = lpmod.gen_kcol(m, n, k, seed)
     train = [lpmod.perturbed_instance(base, sigma, 10000 + s) for s in range(n_train)]
     held = [lpmod.perturbed_instance(base, sigma, 20000 + s) for s in range(warmup + steps)]
-    slack_prm = default_params(refactor_freq=refactor_freq)
+    # The reference's restart loops have no exit; a family solve that runs past slack_limit_s ends
+    # with the iteration-limit status and is reported as not optimal.
+    slack_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=1e3 * slack_limit_s)
     log("learned warm start m=%d n=%d: solving %d training instances, %d at a time" % (m, n, n_train, threads))
     t_tr, tr = _solve_many([(lp, None) for lp in train], lambda i: slack_prm, threads)
-    bases = [r[5] for r in tr]
-    limit_ms = 2.0 * float(np.mean([r[6] for r in tr]))
+    tr_ok = [r for r in tr if r[1] == 0]
+    keep = [i for i, r in enumerate(tr) if r[1] == 0]
+    train = [train[i] for i in keep]
+    bases = [tr[i][5] for i in keep]
+    limit_ms = 2.0 * float(np.mean([r[6] for r in tr_ok]))
     warm_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=limit_ms)
     with BatchSolver(train) as T, Mlp(seed=1) as net:
         T.set_bases(bases)
@@ -450,8 +457,9 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
     wall_w, rw, fallbacks = warm_phase(ev, ew)
     wall_w += pred_s * steps / len(held)
     overlap = [len(set(w.tolist()) & set(r[5].tolist())) / m for w, r in zip(ew, rs)]
-    ok = all(a[1] == 0 and b[1] == 0 for a, b in zip(rs, rw))
-    rel = max(abs(a[2] - b[2]) / max(1.0, abs(a[2])) for a, b in zip(rs, rw))
+    both = [(a, b) for a, b in zip(rs, rw) if a[1] == 0 and b[1] == 0]
+    ok = len(both) == len(rs)
+    rel = max([abs(a[2] - b[2]) / max(1.0, abs(a[2])) for a, b in both] or [0.0])
     out = {
         "workload": "LP family m=%d n=%d, %d nnz/col (base seed %d), b and c x lognormal(0, %g); trained on %d "
                     "GPU-solved instances (seeds 10000+), %d untimed + %d timed held-out instances (seeds 20000+), "
@@ -471,8 +479,9 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
                          "predict_s": round(pred_s, 4)},
         "speedup_solves_per_s": round(wall_s / wall_w, 4),
         "predicted_basis_overlap_with_optimal": round(float(np.mean(overlap)), 4),
-        "all_optimal": bool(ok), "max_obj_rel_err_warm_vs_slack": float(rel),
-        "training": {"instances": n_train, "solve_wall_s": round(t_tr, 2),
+        "all_optimal": bool(ok), "optimal_both": len(both), "slack_not_optimal": sum(a[1] != 0 for a in rs),
+        "max_obj_rel_err_warm_vs_slack": float(rel),
+        "training": {"instances": n_train, "optimal": len(keep), "solve_wall_s": round(t_tr, 2),
                      "mean_iterations": round(float(np.mean([r[3] for r in tr])), 1),
                      "epochs": epochs, "epoch_loss": [round(float(v), 4) for v in losses], "fit_ms": round(fit_ms, 1),
                      "train_gemm_tflops": round(tr_flops / (tr_ms * 1e-3) / 1e12, 2) if tr_ms > 0 else None},

@@ -18,6 +18,8 @@
  *   ds_get_solution   Problem::getSolution / LPsolution    common/problem.tpp:243-259,
  *                                                          common/LPsolution.hpp:22-44
  *   ds_get_basis      basis_indexes (protected in the reference, baseDualSimplex.hpp:80)
+ *   ds_load_mps       Problem::readMps (via Clp) + transformToComputeForm
+ *                                                   common/problem.tpp:331-415, 176-226
  *   ds_refactor       reFactorize                       cpu/sequentialDualSimplex.cpp:155-160
  *   ds_btran          BTran -> solveLinSys(e_p, rho, true)  cpu/sequentialDualSimplex.cpp:89-92
  *   ds_ftran          solveLinSys(b, x, false)          cpu/sequentialDualSimplex.cpp:28-48
@@ -139,6 +141,28 @@ int ds_load_problem(ds_ctx* ctx, int m, int n, const int* col_ptr, const int* ro
                     const int* bound_type, const double* lower_range, const double* upper_range,
                     const int* range_type);
 int ds_set_params(ds_ctx* ctx, const ds_params* p);
+
+/* Problem::readMps + transformToComputeForm (common/problem.tpp:331-415,
+ * 176-226): reads a free- or fixed-format MPS file with this library's own
+ * parser (the reference reads it through COIN-OR Clp) and loads the model as
+ * ds_load_problem does; the objective constant (minus the RHS entry of the
+ * objective row) is removed from Z as getSolution does (problem.tpp:256).
+ * *m and *n (may be NULL) receive the model's size. */
+int ds_load_mps(ds_ctx* ctx, const char* path, int* m, int* n);
+/* Objective constant for a model loaded with ds_load_problem (Problem::_offset). */
+int ds_set_offset(ds_ctx* ctx, double offset);
+
+/* The parsed model itself, for callers that keep their own copy (host only, no
+ * GPU needed): ds_mps_read parses (on failure err receives the message),
+ * ds_mps_dims gives m, n and nnz, ds_mps_get copies the 8 arrays of the
+ * reference's Problem constructor (A in CSC) and the objective constant. */
+typedef struct ds_mps ds_mps;
+int ds_mps_read(const char* path, ds_mps** out, char* err, int err_len);
+int ds_mps_dims(const ds_mps* mps, int* m, int* n, int* nnz);
+int ds_mps_get(const ds_mps* mps, int* col_ptr, int* row_idx, double* val, double* costs, double* lower_bound,
+               double* upper_bound, int* bound_type, double* lower_range, double* upper_range, int* range_type,
+               double* offset);
+int ds_mps_free(ds_mps* mps);
 int ds_init(ds_ctx* ctx);
 /* ds_init from a given starting basis (m variable ids; the nonbasic set is the
  * ascending complement) instead of the slack basis.  No counterpart in the

@@ -68,6 +68,12 @@ _SIGS = {
     "ds_set_params": (I32, [VP, C.POINTER(Params)]),
     "ds_init": (I32, [VP]),
     "ds_init_basis": (I32, [VP, VP]),
+    "ds_load_mps": (I32, [VP, C.c_char_p, C.POINTER(I32), C.POINTER(I32)]),
+    "ds_set_offset": (I32, [VP, F64]),
+    "ds_mps_read": (I32, [C.c_char_p, C.POINTER(VP), C.c_char_p, I32]),
+    "ds_mps_dims": (I32, [VP, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "ds_mps_get": (I32, [VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.POINTER(F64)]),
+    "ds_mps_free": (I32, [VP]),
     "ds_presolve": (I32, [VP, C.c_char_p, C.POINTER(I32)]),
     "ds_solve": (I32, [VP]),
     "ds_get_result": (I32, [VP, C.POINTER(Result)]),

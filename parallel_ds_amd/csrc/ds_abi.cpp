@@ -1,5 +1,6 @@
 // extern "C" boundary of libpds.so: argument checks, exception-to-error-code
 // translation, and the host/device copies of the hook-level entries.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -11,6 +12,7 @@
 #include "ds_warm.hpp"
 
 #include "ds_ctx.hpp"
+#include "ds_mps.hpp"
 
 namespace pds {
 int chuzr_noalloc_setup(int device);  // ds_chuzr.hip
@@ -175,6 +177,87 @@ int ds_init(ds_ctx* ctx) {
 int ds_time_refactor(ds_ctx* ctx, int reps, double* refactor_ms, double* gemm_ms, double* gemm_flops, int* k) {
   if (reps <= 0 || !refactor_ms || !gemm_ms || !gemm_flops || !k) return DS_EINVAL;
   return guard(ctx, [&](pds::Solver& s) { s.time_refactor(reps, *refactor_ms, *gemm_ms, *gemm_flops, *k); });
+}
+
+int ds_load_mps(ds_ctx* ctx, const char* path, int* m, int* n) {
+  if (!path) return DS_EINVAL;
+  return guard(ctx, [&](pds::Solver& s) {
+    pds::MpsModel M;
+    try {
+      pds::read_mps(path, M);
+    } catch (const std::runtime_error& e) {
+      throw std::invalid_argument(e.what());
+    }
+    pds::HostModel hm;
+    hm.build(M.m, M.n, M.col_ptr.data(), M.row_idx.data(), M.val.data(), M.costs.data(), M.lower_bound.data(),
+             M.upper_bound.data(), M.bound_type.data(), M.lower_range.data(), M.upper_range.data(),
+             M.range_type.data());
+    hm.offset = M.offset;
+    s.load(hm);
+    if (m) *m = M.m;
+    if (n) *n = M.n;
+  });
+}
+
+int ds_set_offset(ds_ctx* ctx, double offset) {
+  return guard(ctx, [&](pds::Solver& s) { s.set_offset(offset); });
+}
+
+struct ds_mps {
+  pds::MpsModel M;
+};
+
+int ds_mps_read(const char* path, ds_mps** out, char* err, int err_len) {
+  if (!path || !out) return DS_EINVAL;
+  *out = nullptr;
+  ds_mps* h = new ds_mps();
+  try {
+    pds::read_mps(path, h->M);
+  } catch (const std::exception& e) {
+    if (err && err_len > 0) {
+      std::strncpy(err, e.what(), err_len - 1);
+      err[err_len - 1] = '\0';
+    }
+    delete h;
+    return DS_EINVAL;
+  }
+  *out = h;
+  return DS_OK;
+}
+
+int ds_mps_dims(const ds_mps* mps, int* m, int* n, int* nnz) {
+  if (!mps) return DS_EINVAL;
+  if (m) *m = mps->M.m;
+  if (n) *n = mps->M.n;
+  if (nnz) *nnz = (int)mps->M.val.size();
+  return DS_OK;
+}
+
+int ds_mps_get(const ds_mps* mps, int* col_ptr, int* row_idx, double* val, double* costs, double* lower_bound,
+               double* upper_bound, int* bound_type, double* lower_range, double* upper_range, int* range_type,
+               double* offset) {
+  if (!mps) return DS_EINVAL;
+  const pds::MpsModel& M = mps->M;
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  cp(col_ptr, M.col_ptr);
+  cp(row_idx, M.row_idx);
+  cp(val, M.val);
+  cp(costs, M.costs);
+  cp(lower_bound, M.lower_bound);
+  cp(upper_bound, M.upper_bound);
+  cp(bound_type, M.bound_type);
+  cp(lower_range, M.lower_range);
+  cp(upper_range, M.upper_range);
+  cp(range_type, M.range_type);
+  if (offset) *offset = M.offset;
+  return DS_OK;
+}
+
+int ds_mps_free(ds_mps* mps) {
+  delete mps;
+  return DS_OK;
 }
 
 int ds_init_basis(ds_ctx* ctx, const int* basis) {

@@ -96,6 +96,7 @@ class Solver {
 
   std::string last_error;
   hipStream_t stream() const { return st_; }
+  void set_offset(double offset) { H_.offset = offset; }
 
  private:
   int device_ = 0;

@@ -26,6 +26,7 @@ class LP:
     upper_range: np.ndarray
     range_type: np.ndarray
     name: str = ""
+    offset: float = 0.0   # objective constant (Problem::_offset), removed from Z
 
     @property
     def nnz(self):
@@ -166,3 +167,93 @@ def boxed_instance(m, n, density, seed):
     return LP(m, n, base.col_ptr, base.row_idx, base.val, base.costs, np.zeros(n), ub,
               np.full(n, DS_BOXED, np.int32), 0.2 * b, b.copy(), np.full(m, DS_BOXED, np.int32),
               "boxed-m%d-n%d-s%d" % (m, n, seed))
+
+
+def read_mps(path):
+    """Problem::readMps (src/common/problem.tpp:331-415) with this library's MPS parser: the
+    8 arrays of the model before transformToComputeForm, and the objective constant."""
+    import ctypes as C
+    from ._native import DsError
+    L = lib()
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = L.ds_mps_read(str(path).encode(), C.byref(h), err, 512)
+    if rc != 0:
+        raise DsError(rc, "ds_mps_read", err.value.decode())
+    try:
+        m, n, nnz = C.c_int(), C.c_int(), C.c_int()
+        L.ds_mps_dims(h, C.byref(m), C.byref(n), C.byref(nnz))
+        m, n, nnz = m.value, n.value, nnz.value
+        cp = np.zeros(n + 1, np.int32)
+        ri = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        c, lb, ub = np.zeros(n), np.zeros(n), np.zeros(n)
+        bt = np.zeros(n, np.int32)
+        lr, ur = np.zeros(m), np.zeros(m)
+        rt = np.zeros(m, np.int32)
+        off = C.c_double()
+        L.ds_mps_get(h, ptr(cp), ptr(ri), ptr(v), ptr(c), ptr(lb), ptr(ub), ptr(bt), ptr(lr), ptr(ur), ptr(rt),
+                     C.byref(off))
+    finally:
+        L.ds_mps_free(h)
+    import os
+    return LP(m, n, cp, ri[:nnz].copy(), v[:nnz].copy(), c, lb, ub, bt, lr, ur, rt,
+              os.path.basename(str(path)), off.value)
+
+
+def write_mps(lp, path, name="LP"):
+    """Free-format MPS of an LP (test and interchange helper): rows E/L/G with RANGES for ranged
+    rows, BOUNDS for every column whose bounds differ from [0, +inf)."""
+    inf = 1e30
+    with open(path, "w") as f:
+        f.write("NAME %s\nROWS\n N COST\n" % name)
+        kind = []
+        for i in range(lp.m):
+            lo, up = lp.lower_range[i], lp.upper_range[i]
+            if np.isfinite(lo) and np.isfinite(up) and lo == up:
+                kind.append("E")
+            elif np.isfinite(up):
+                kind.append("L")
+            elif np.isfinite(lo):
+                kind.append("G")
+            else:
+                raise ValueError("free rows are not written")
+            f.write(" %s R%d\n" % (kind[-1], i))
+        f.write("COLUMNS\n")
+        for j in range(lp.n):
+            if lp.costs[j] != 0.0:
+                f.write(" X%d COST %.17g\n" % (j, lp.costs[j]))
+            for e in range(lp.col_ptr[j], lp.col_ptr[j + 1]):
+                f.write(" X%d R%d %.17g\n" % (j, lp.row_idx[e], lp.val[e]))
+            if lp.costs[j] == 0.0 and lp.col_ptr[j] == lp.col_ptr[j + 1]:
+                f.write(" X%d COST 0\n" % j)
+        f.write("RHS\n")
+        for i in range(lp.m):
+            rhs = lp.lower_range[i] if kind[i] == "G" else lp.upper_range[i]
+            if rhs != 0.0:
+                f.write(" RHS R%d %.17g\n" % (i, rhs))
+        if lp.offset != 0.0:
+            f.write(" RHS COST %.17g\n" % (-lp.offset))
+        rng = [(i, lp.upper_range[i] - lp.lower_range[i]) for i in range(lp.m)
+               if kind[i] != "E" and np.isfinite(lp.lower_range[i]) and np.isfinite(lp.upper_range[i])]
+        if rng:
+            f.write("RANGES\n")
+            for i, r in rng:
+                f.write(" RNG R%d %.17g\n" % (i, r))
+        f.write("BOUNDS\n")
+        for j in range(lp.n):
+            lo, up = lp.lower_bound[j], lp.upper_bound[j]
+            if not np.isfinite(lo) and not np.isfinite(up):
+                f.write(" FR BND X%d\n" % j)
+                continue
+            if np.isfinite(lo) and np.isfinite(up) and lo == up:
+                f.write(" FX BND X%d %.17g\n" % (j, lo))
+                continue
+            if not np.isfinite(lo):
+                f.write(" MI BND X%d\n" % j)
+            elif lo != 0.0:
+                f.write(" LO BND X%d %.17g\n" % (j, lo))
+            if np.isfinite(up):
+                f.write(" UP BND X%d %.17g\n" % (j, up))
+        f.write("ENDATA\n")
+    _ = inf

@@ -35,15 +35,30 @@ def default_params(**kw):
     return p
 
 
+class _Size:
+    def __init__(self, m, n):
+        self.m, self.n = m, n
+
+
 class DualSimplex:
+    """`lp` is an LP of parallel_ds_amd.lp (the 8 arrays of Problem's constructor) or the path of
+    an MPS file (Problem::readMps, read by the library's parser)."""
+
     def __init__(self, lp, params=None, device=0):
         L = lib()
         self._ctx = C.c_void_p()
         rc = L.ds_create(device, C.byref(self._ctx))
         if rc != 0:
             raise DsError(rc, "ds_create", "(no usable GPU?)")
-        self.lp = lp
-        self._check(L.ds_load_problem(self._ctx, lp.m, lp.n, *[ptr(a) for a in lp.arrays()]), "ds_load_problem")
+        if isinstance(lp, (str, bytes)) or hasattr(lp, "__fspath__"):
+            m, n = C.c_int(), C.c_int()
+            self._check(L.ds_load_mps(self._ctx, str(lp).encode(), C.byref(m), C.byref(n)), "ds_load_mps")
+            self.lp = _Size(m.value, n.value)
+        else:
+            self.lp = lp
+            self._check(L.ds_load_problem(self._ctx, lp.m, lp.n, *[ptr(a) for a in lp.arrays()]), "ds_load_problem")
+            if getattr(lp, "offset", 0.0):
+                self._check(L.ds_set_offset(self._ctx, float(lp.offset)), "ds_set_offset")
         self.params = params if params is not None else default_params()
         self._check(L.ds_set_params(self._ctx, C.byref(self.params)), "ds_set_params")
 
