@@ -222,10 +222,19 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   __syncthreads();
   for (int k = tid; k < t; k += BS) {
     double acc = 0.0;
-    if (tile)
+    if (tile) {
       for (int k2 = k; k2 < t; ++k2) acc += mf[k2 * MF_TILE + k] * bv[k2];
-    else
-      for (int k2 = k; k2 < t; ++k2) acc += I.MF[(size_t)k2 * cap + k] * bv[k2];
+    } else {
+      // four independent partial sums keep four MF loads in flight
+      double a4[4] = {0.0, 0.0, 0.0, 0.0};
+      int k2 = k;
+      for (; k2 + 3 < t; k2 += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a4[q] += I.MF[(size_t)(k2 + q) * cap + k] * bv[k2 + q];
+      }
+      for (; k2 < t; ++k2) a4[0] += I.MF[(size_t)k2 * cap + k] * bv[k2];
+      acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+    }
     u[k] = acc;
   }
   __syncthreads();
@@ -308,45 +317,62 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   }
 }
 
+// A workgroup takes BO_ROWS consecutive rows; its BS / BO_ROWS row groups
+// split the right-hand-side list between them (each a contiguous quarter), so
+// the serial walk over the list per thread is a quarter as long, and the
+// partial sums meet in LDS.
+constexpr int BO_ROWS = 64;
 __global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFactor F, double* out, int check) {
   DevScalars* S = I.S;
   if (skip_launch(S, check)) return;
-  __shared__ int sidx[1024];
-  __shared__ double sval[1024];
+  constexpr int NG = BS / BO_ROWS;
+  constexpr int CH = 1024;
+  __shared__ int sidx[CH];
+  __shared__ double sval[CH];
   __shared__ int yrow[ETA_MAX + 1];
   __shared__ double yval[ETA_MAX + 1];
+  __shared__ double part[NG][BO_ROWS];
   const int total = I.list_n[0];
   const int ny = I.list_n[1];
   for (int s = threadIdx.x; s < ny; s += BS) {
     yrow[s] = I.ylist_row[s];
     yval[s] = I.ylist_val[s];
   }
-  const int r = blockIdx.x * BS + threadIdx.x;
+  const int g = threadIdx.x / BO_ROWS, lr = threadIdx.x % BO_ROWS;
+  const int r = blockIdx.x * BO_ROWS + lr;
   const bool live = r < M.m;
   const int ri = live ? F.Ridx_of_row[r] : -1;
   double acc = 0.0;
-  for (int base = 0; base < total; base += 1024) {
-    int nchunk = min(1024, total - base);
+  for (int base = 0; base < total; base += CH) {
+    const int nchunk = min(CH, total - base);
     __syncthreads();
     for (int e = threadIdx.x; e < nchunk; e += BS) {
       sidx[e] = I.list_idx[base + e];
       sval[e] = I.list_val[base + e];
     }
     __syncthreads();
-    if (live && ri >= 0)
-      for (int e = 0; e < nchunk; ++e) {
-        int c = sidx[e];
-        if (c >= 0) acc += sval[e] * F.Kinv[(size_t)c * F.k + ri];
+    const int per = (nchunk + NG - 1) / NG;
+    const int e0 = g * per, e1 = min(nchunk, e0 + per);
+    if (live && ri >= 0) {
+      const double* col = F.Kinv + ri;
+      for (int e = e0; e < e1; ++e) {
+        const int c = sidx[e];
+        if (c >= 0) acc += sval[e] * col[(size_t)c * F.k];
       }
+    }
   }
+  part[g][lr] = acc;
   __syncthreads();
-  if (!live) return;
+  if (g != 0 || !live) return;
+  double sum = part[0][lr];
+#pragma unroll
+  for (int q = 1; q < NG; ++q) sum += part[q][lr];
   if (ri < 0) {
-    acc = 0.0;
-    for (int s = 0; s < ny; ++s)
-      if (yrow[s] == r) acc = yval[s];
+    sum = 0.0;
+    for (int s2 = 0; s2 < ny; ++s2)
+      if (yrow[s2] == r) sum = yval[s2];
   }
-  out[r] = acc;
+  out[r] = sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -741,20 +767,45 @@ __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const d
     xm[k] = (I.eta_last[k] < 0) ? x0[q] : 0.0;
   }
   __syncthreads();
+  // s = MF xm: one thread per row of MF, its row read in 16-byte pairs with
+  // four independent partial sums, so that many loads are in flight per thread.
   for (int k = threadIdx.x; k < t; k += BS) {
-    const double* row = I.MF + (size_t)k * cap;
-    double acc = 0.0;
-    for (int j = 0; j <= k; ++j) acc += row[j] * xm[j];
-    sk[k] = acc;
+    const double* row = I.MF + (size_t)k * cap;  // 16-byte aligned: cap is even
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int j = 0;
+    for (; j + 3 <= k; j += 4) {
+      const double2 r0 = *reinterpret_cast<const double2*>(row + j);
+      const double2 r1 = *reinterpret_cast<const double2*>(row + j + 2);
+      a0 += r0.x * xm[j];
+      a1 += r0.y * xm[j + 1];
+      a2 += r1.x * xm[j + 2];
+      a3 += r1.y * xm[j + 3];
+    }
+    for (; j <= k; ++j) a0 += row[j] * xm[j];
+    sk[k] = (a0 + a1) + (a2 + a3);
   }
   __syncthreads();
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= M.m) return;
   double x = x0[i];
-  for (int k = 0; k < t; ++k) {
-    double s = sk[k];
-    if (pk[k] == i) x = s * I.etas[(size_t)k * M.m + i];
-    else if (s != 0.0) x = x + s * I.etas[(size_t)k * M.m + i];
+  // Eight eta entries are loaded ahead of their (order-dependent) application.
+  const double* ei = I.etas + i;
+  int k = 0;
+  for (; k + 8 <= t; k += 8) {
+    double e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) e[q] = ei[(size_t)(k + q) * M.m];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double s = sk[k + q];
+      if (pk[k + q] == i) x = s * e[q];
+      else if (s != 0.0) x = x + s * e[q];
+    }
+  }
+  for (; k < t; ++k) {
+    const double s = sk[k];
+    if (pk[k] == i) x = s * ei[(size_t)k * M.m];
+    else if (s != 0.0) x = x + s * ei[(size_t)k * M.m];
   }
   if (mode == 0) out[i] = x;
   else out[I.basis[i]] = x;
@@ -1443,7 +1494,7 @@ void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed) {
   k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, p_fixed);
-  k_btran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0);
+  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0);
 }
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
                  const double* cvec, double* out, int by_col, const DevScalars* S, int check, hipEvent_t ev0,
