@@ -83,6 +83,14 @@ typedef struct {
   int force_at;
   double time_limit_ms;   /* > 0: end a solve that has run this long (phase 1 + phase 2 wall time) with
                              DS_ITERATION_LIMIT and the message "time limit"; 0 = none (reference) */
+  int candidates;         /* phase 2 leaving-row candidates per major iteration, 1..8; 1 (default) = the
+                             sequential solver.  > 1: the minor iterations of the reference's parallel
+                             dual simplex (parallelDualSimplex.cpp:431-485, 747-1124) on one GPU: the
+                             major pricing keeps the rows of largest delta^2/beta, their BTRANs run back to
+                             back, each minor iteration takes the best still-eligible row with no BTRAN of
+                             its own, and the others' rows follow each basis change */
+  double candidate_psi;   /* a candidate drops out when its weight falls below psi times its previous
+                             one (the reference's psi, main.cpp:46-53); default 0.9 */
 } ds_params;
 
 /* Recovery paths a test can force from the host (sequentialDualSimplex.cpp):

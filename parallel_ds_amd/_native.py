@@ -20,7 +20,7 @@ class Params(C.Structure):
                 ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
                 ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
                 ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int),
-                ("time_limit_ms", C.c_double)]
+                ("time_limit_ms", C.c_double), ("candidates", C.c_int), ("candidate_psi", C.c_double)]
 
 
 DS_FORCE_NONE, DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART, DS_FORCE_SINGULAR = 0, 1, 2, 3

@@ -105,6 +105,8 @@ void ds_default_params(ds_params* p) {
   p->force_event = DS_FORCE_NONE;
   p->force_at = 0;
   p->time_limit_ms = 0.0;
+  p->candidates = 1;
+  p->candidate_psi = 0.9;
 }
 
 int ds_create(int device, ds_ctx** out) {

@@ -38,7 +38,8 @@ enum : int {
   ST_PH1_REINIT = 9,       // phase 1: cycle > NEED_RESTART or Z blew up
   ST_BFRT = 10,            // (unused: bound flips run on the device)
   ST_BTRAN_CAP = 11,       // sparse BTRAN list overflowed its LDS capacity
-  ST_ERROR = 12
+  ST_ERROR = 12,
+  ST_MAJOR_END = 13        // multiple candidates: no candidate row left in this major iteration
 };
 
 // Scalars of one iteration, resident in device memory.
@@ -60,6 +61,8 @@ struct DevScalars {
   int k;             // kernel size of the current factor
   int bfrt;          // the entering candidate is boxed and flips: the bound-flipping loop runs
   int nflip;         // boxed nonbasic variables that flip in this iteration
+  int ncand;         // multiple candidates: rows chosen by the last major pricing
+  int cand_best;     // the candidate the current minor iteration takes
   int pad0;
   double delta, theta, obj, f_p, corr, alpha_p, alpha_rq, theta_P;
   double tmp0, tmp1;
@@ -71,5 +74,7 @@ constexpr int MAX_BLOCKED = 64;
 
 // Counter slots in DevScalars::ctr.
 enum { CTR_PRICE = 0, CTR_CHUZC1, CTR_CHUZC2, CTR_DUPD, CTR_UPD, CTR_DCHK, CTR_DOT, CTR_INF, CTR_SPMV, CTR_FLIP, CTR_BXB };
+
+constexpr int CAND_MAX = 8;  // candidate rows per major iteration
 
 }  // namespace pds

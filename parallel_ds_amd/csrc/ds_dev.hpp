@@ -51,6 +51,11 @@ struct DevIter {
   int* flist;              // n: flipping positions in ascending order
   double* bcc;             // m: column change sum_j (+-)(u_j - l_j) a_j of the flips
   double* dxb;             // m: B^{-1} bcc
+  // Multiple candidate rows per major iteration (params.candidates > 1):
+  int* cand_pos;           // CAND_MAX basis positions
+  double* cand_w;          // their last weights delta^2 / beta
+  int* cand_act;           // still eligible
+  double* rho_c;           // CAND_MAX x m: their BTRAN rows, kept current through the minor iterations
   int eta_cap;
   unsigned char* blocked;  // N
   int* blocked_list;       // MAX_BLOCKED

@@ -155,10 +155,12 @@ __global__ __launch_bounds__(BS) void k_dualcheck(DevModel M, DevIter I, int it)
 // entry).  The kernel-inverse gather then runs on all rows in k_btran_out.
 // ---------------------------------------------------------------------------
 constexpr int MF_TILE = 64;  // eta counts up to this keep MF's lower triangle in LDS
-__global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed) {
+__global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed,
+                                                   int cand) {
   DevScalars* S = I.S;
   if (p_fixed < 0 && stopped(S)) return;
-  const int p = p_fixed >= 0 ? p_fixed : S->p_idx;
+  if (cand >= 0 && cand >= S->ncand) return;
+  const int p = cand >= 0 ? I.cand_pos[cand] : (p_fixed >= 0 ? p_fixed : S->p_idx);
   const int m = M.m;
   const int cap = I.eta_cap;
   __shared__ int pos[ETA_MAX + 1];
@@ -322,9 +324,11 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
 // the serial walk over the list per thread is a quarter as long, and the
 // partial sums meet in LDS.
 constexpr int BO_ROWS = 64;
-__global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFactor F, double* out, int check) {
+__global__ __launch_bounds__(BS) void k_btran_out(DevModel M, DevIter I, DevFactor F, double* out, int check,
+                                                  int cand) {
   DevScalars* S = I.S;
   if (skip_launch(S, check)) return;
+  if (cand >= 0 && cand >= S->ncand) return;
   constexpr int NG = BS / BO_ROWS;
   constexpr int CH = 1024;
   __shared__ int sidx[CH];
@@ -1045,6 +1049,160 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
 }
 
 // ---------------------------------------------------------------------------
+// Multiple candidate rows per major iteration (the minor iterations of the
+// reference's parallel dual simplex, parallelDualSimplex.cpp:431-485,
+// :747-1124, on one GPU): the major pricing keeps the `nc` rows of largest
+// delta^2/beta, their BTRAN rows are computed back to back, and each minor
+// iteration takes the best still-eligible candidate without a BTRAN of its
+// own; after each basis change the other candidates' rows are brought up to
+// date with the product-form row update.
+// ---------------------------------------------------------------------------
+// Major pricing: the top-of-iteration checks and the optimality test of
+// k_price (phase 2), then the nc best rows (weight, then lowest position).
+__global__ __launch_bounds__(BS) void k_price_multi(DevModel M, DevIter I, int it, int skip_top, int perturb_on,
+                                                    int nc) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  if (!skip_top && perturb_on && !S->perturbed && S->cycle > MAX_CYCLE) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) raise_status(S, ST_NEED_PERTURB, it);
+    return;
+  }
+  __shared__ int taken[CAND_MAX];
+  int pinf = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    const int j = I.basis[i];
+    pinf |= primal_infeas(M.bt[j], I.x[j], M.l[j], M.u[j]) ? 1 : 0;
+  }
+  pinf = block_sumi<BS>(pinf);
+  // this workgroup's nc best, one argmax round per candidate
+  for (int c = 0; c < nc; ++c) {
+    double bw = -1.0;
+    int bi = INT_MAX;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+      bool used = false;
+      for (int c2 = 0; c2 < c; ++c2) used |= taken[c2] == i;
+      if (used) continue;
+      const int j = I.basis[i];
+      double dl;
+      if (var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl) && fabs(dl) > DSE_FLOOR && !I.blocked[j]) {
+        const double w = (dl * dl) / I.beta[i];
+        if (w > 0.0) argmax_merge(bw, bi, w, i);
+      }
+    }
+    block_argmax<BS>(bw, bi);
+    if (threadIdx.x == 0) {
+      taken[c] = bi;
+      I.part_d[blockIdx.x * CAND_MAX + c] = bw;
+      I.part_i[blockIdx.x * CAND_MAX + c] = bi;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) I.part_i[4096 + blockIdx.x] = pinf;
+  if (!last_block(&S->ctr[CTR_PRICE])) return;
+  int fp = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) fp += I.part_i[4096 + b];
+  fp = block_sumi<BS>(fp);
+  const int total = gridDim.x * nc;
+  int got = 0;
+  for (int c = 0; c < nc; ++c) {
+    double bw = -1.0;
+    int bi = INT_MAX;
+    for (int e = threadIdx.x; e < total; e += BS) {
+      const int b = e / nc, c2 = e % nc;
+      const int i = I.part_i[b * CAND_MAX + c2];
+      if (i == INT_MAX) continue;
+      bool used = false;
+      for (int c3 = 0; c3 < c; ++c3) used |= taken[c3] == i;
+      if (!used) argmax_merge(bw, bi, I.part_d[b * CAND_MAX + c2], i);
+    }
+    block_argmax<BS>(bw, bi);
+    if (threadIdx.x == 0) {
+      taken[c] = bi;
+      if (bi != INT_MAX) {
+        const int j = I.basis[bi];
+        double dl;
+        var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl);
+        I.cand_pos[c] = bi;
+        I.cand_w[c] = bw;
+        I.cand_act[c] = dl < 0 ? 2 : 1;  // eligible, and on which side of its bounds
+      }
+    }
+    __syncthreads();
+    if (bi != INT_MAX) got = c + 1;
+  }
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_PRICE] = 0;
+  S->ncand = got;
+  if (!fp) raise_status(S, ST_OPTIMAL, it);
+  else if (got == 0) raise_status(S, ST_NO_PIVOT, it);
+}
+
+// Minor pricing (one workgroup): each eligible candidate's current
+// infeasibility; it drops out when its row has become feasible or changed
+// side, or when its weight fell below psi times its previous one
+// (parallelDualSimplex.cpp:842-874).  The best remaining one is this
+// iteration's leaving row, and its BTRAN row becomes rho.
+__global__ __launch_bounds__(BS) void k_pick(DevModel M, DevIter I, int it, double psi) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  __shared__ int s_best;
+  if (threadIdx.x == 0) {
+    double bw = -1.0;
+    int best = -1;
+    double bdl = 0.0;
+    for (int c = 0; c < S->ncand; ++c) {
+      if (!I.cand_act[c]) continue;
+      const int pos = I.cand_pos[c];
+      const int j = I.basis[pos];
+      double dl;
+      const bool inf = var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl) && fabs(dl) > DSE_FLOOR;
+      const double w = inf ? (dl * dl) / I.beta[pos] : 0.0;
+      const bool same_side = (I.cand_act[c] == 2) == (dl < 0);  // 2: it was below its lower bound
+      if (!inf || !same_side || w < psi * I.cand_w[c]) {
+        I.cand_act[c] = 0;
+        continue;
+      }
+      I.cand_w[c] = w;
+      if (w > bw) {
+        bw = w;
+        best = c;
+        bdl = dl;
+      }
+    }
+    s_best = best;
+    if (best < 0) {
+      raise_status(S, ST_MAJOR_END, it);
+    } else {
+      const int pos = I.cand_pos[best];
+      I.cand_act[best] = 0;
+      S->cand_best = best;
+      S->p_idx = pos;
+      S->p = I.basis[pos];
+      S->delta = bdl;
+      S->is_lower = bdl < 0 ? 1 : 0;
+      S->neg = S->is_lower;
+    }
+  }
+  __syncthreads();
+  if (s_best < 0) return;
+  const double* src = I.rho_c + (size_t)s_best * M.m;
+  for (int r = threadIdx.x; r < M.m; r += BS) I.rho[r] = src[r];
+}
+
+// After the basis change at row p: row c of the new inverse is
+// row c - (alpha_q[c] / alpha_q[p]) row p, for every still-eligible candidate.
+__global__ __launch_bounds__(BS) void k_rho_update(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int c = blockIdx.y;
+  if (c >= S->ncand || !I.cand_act[c]) return;
+  const int pc = I.cand_pos[c], p = S->p_idx;
+  const double f = I.alpha_q[pc] / I.alpha_q[p];
+  double* rc = I.rho_c + (size_t)c * M.m;
+  for (int r = blockIdx.x * BS + threadIdx.x; r < M.m; r += gridDim.x * BS) rc[r] = rc[r] - f * I.rho[r];
+}
+
+// ---------------------------------------------------------------------------
 // Bound-flipping ratio test and BFRT FTRAN (sequentialDualSimplex.cpp:398-482),
 // queued only for models with boxed variables.
 // ---------------------------------------------------------------------------
@@ -1493,8 +1651,12 @@ void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it
 }
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed) {
-  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, p_fixed);
-  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0);
+  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, p_fixed, -1);
+  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0, -1);
+}
+void launch_btran_cand(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, int cand) {
+  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, -1, cand);
+  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, I.rho_c + (size_t)cand * M.m, 1, cand);
 }
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
                  const double* cvec, double* out, int by_col, const DevScalars* S, int check, hipEvent_t ev0,
@@ -1569,6 +1731,16 @@ void launch_force(hipStream_t s, const DevModel& M, const DevIter& I, int event)
 }
 void launch_force_zero_column(hipStream_t s, double* W, int k, int c) {
   if (k > 0 && c >= 0 && c < k) k_force_zero_column<<<(k + 255) / 256, 256, 0, s>>>(W, k, c);
+}
+void launch_price_multi(hipStream_t s, const DevModel& M, const DevIter& I, int it, int skip_top, int perturb_on,
+                        int nc) {
+  k_price_multi<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it, skip_top, perturb_on, nc);
+}
+void launch_pick(hipStream_t s, const DevModel& M, const DevIter& I, int it, double psi) {
+  k_pick<<<1, BS, 0, s>>>(M, I, it, psi);
+}
+void launch_rho_update(hipStream_t s, const DevModel& M, const DevIter& I, int nc) {
+  k_rho_update<<<dim3(cap_grid(M.m, BS, 64), nc), BS, 0, s>>>(M, I);
 }
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I) {
   k_set_primal<<<nblk(M.N - M.m, BS), BS, 0, s>>>(M, I);

@@ -32,6 +32,12 @@ void launch_flips(hipStream_t s, const DevModel& M, const DevIter& I, const DevF
 // iteration chain, and a zeroed kernel column before a factorization.
 void launch_force(hipStream_t s, const DevModel& M, const DevIter& I, int event);
 void launch_force_zero_column(hipStream_t s, double* W, int k, int c);
+// Multiple candidate rows per major iteration (params.candidates > 1).
+void launch_price_multi(hipStream_t s, const DevModel& M, const DevIter& I, int it, int skip_top, int perturb_on,
+                        int nc);
+void launch_btran_cand(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, int cand);
+void launch_pick(hipStream_t s, const DevModel& M, const DevIter& I, int it, double psi);
+void launch_rho_update(hipStream_t s, const DevModel& M, const DevIter& I, int nc);
 void launch_set_primal(hipStream_t s, const DevModel& M, const DevIter& I);
 void launch_mask_nonbasic(hipStream_t s, const DevModel& M, const DevIter& I, const double* src, double* wN);
 void launch_rowspmv(hipStream_t s, const DevModel& M, const double* w, const double* b, double alpha, double beta,

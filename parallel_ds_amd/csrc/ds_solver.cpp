@@ -108,6 +108,9 @@ void Solver::set_params(const ds_params& p) {
   if (P_.refactor_freq <= 0) P_.refactor_freq = 150;
   if (P_.refactor_freq > ETA_MAX - 8) throw std::invalid_argument("refactor_freq exceeds the eta file capacity");
   if (P_.chunk <= 0) P_.chunk = 1;
+  if (P_.candidates < 1 || P_.candidates > CAND_MAX) throw std::invalid_argument("candidates must be 1..8");
+  if (P_.candidate_psi < 0.0 || P_.candidate_psi > 1.0) throw std::invalid_argument("candidate_psi must be in [0, 1]");
+  if (P_.candidates > 1 && loaded_ && rho_c_.n < (size_t)CAND_MAX * H_.m) rho_c_.alloc((size_t)CAND_MAX * H_.m);
   if (P_.time_kernels && (int)ev_.size() < 2 * P_.chunk) {
     for (int i = (int)ev_.size(); i < 2 * P_.chunk; ++i) {
       hipEvent_t e;
@@ -233,6 +236,10 @@ DevIter Solver::I() const {
   it.flist = flist_.p;
   it.bcc = bcc_.p;
   it.dxb = dxb_.p;
+  it.cand_pos = cand_pos_.p;
+  it.cand_w = cand_w_.p;
+  it.cand_act = cand_act_.p;
+  it.rho_c = rho_c_.p;
   it.eta_cap = eta_cap_;
   it.blocked = blocked_.p;
   it.blocked_list = blocked_list_.p;
@@ -378,6 +385,10 @@ void Solver::load(const HostModel& hm) {
   flist_.alloc(nb);
   bcc_.alloc(mb);
   dxb_.alloc(mb);
+  cand_pos_.alloc(CAND_MAX);
+  cand_act_.alloc(CAND_MAX);
+  cand_w_.alloc(CAND_MAX);
+  rho_c_.alloc(P_.candidates > 1 ? (size_t)CAND_MAX * m : 1);
   PDS_HIP(hipMemsetAsync(bmark_.p, 0, nb, st_));
   PDS_HIP(hipMemsetAsync(blocked_.p, 0, N, st_));
   PDS_HIP(hipMemsetAsync(E_.p, 0, sizeof(double) * eta_cap_ * eta_cap_, st_));
@@ -678,6 +689,7 @@ void Solver::initDualSimplex() {
   force_fired_ = false;
   force_enq_it_ = -1;
   time_hit_ = false;
+  last_dualcheck_ = 0;
   for (double& v : step_ms_) v = 0.0;
   pev_part_.clear();
   solve_ms_ = refactor_ms_ = 0.0;
@@ -737,12 +749,69 @@ void Solver::enqueue_stageB(int phase, int it, int t) {
   prof_event(DS_PART_BASIS_UPDATE);
   launch_ftran_dense(st_, M(), I(), F(), rho_.p, tau_.p, t, 0, 1);
   launch_update(st_, M(), I(), phase, it);
+  if (phase == 2 && P_.candidates > 1) launch_rho_update(st_, M(), I(), P_.candidates);
   prof_event(DS_PART_BASIS_UPDATE);
   prof_harvest();
 }
 
 // Queue iterations next, next+1, ... up to the chunk size, stopping before
 // the next scheduled refactor and at the iteration cap.
+// One minor iteration of a major iteration with several candidate rows: the
+// best eligible candidate's BTRAN row replaces the BTRAN of the iteration;
+// after the basis change the other candidates' rows are updated.
+void Solver::enqueue_minor(int it, int t) {
+  const DevModel m = M();
+  const DevIter i = I();
+  const DevFactor f = F();
+  prof_event(DS_PART_PRICING);
+  launch_pick(st_, m, i, it, P_.candidate_psi);
+  prof_event(DS_PART_PRICING);
+  launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
+  prof_event(DS_PART_PIVOT_ROW);
+  launch_chuzc(st_, m, i, 2, P_.ratio_test, it);
+  if (has_boxed_) launch_bfrt(st_, m, i, f, P_.ratio_test, t);
+  prof_event(DS_PART_RATIO_TEST2);
+  launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
+  prof_event(DS_PART_FTRAN);
+  launch_ftran_dense(st_, m, i, f, rho_.p, tau_.p, t, 0, 1);
+  prof_event(DS_PART_FTRAN_BETA);
+  if (has_boxed_) {
+    launch_flips(st_, m, i, f, t);
+    prof_event(DS_PART_FTRAN_BFRT);
+  }
+  launch_dupdate_update(st_, m, i, 2, it);
+  launch_rho_update(st_, m, i, P_.candidates);
+  prof_event(DS_PART_UPDATE_RED_COSTS);
+}
+
+// A major iteration (phase 2, several candidates): the dual-feasibility check
+// when a multiple of NEED_RESTART iterations has passed since the last one,
+// the major pricing, the candidates' BTRANs back to back, then up to
+// `candidates` minor iterations, stopping before the next scheduled refactor.
+int Solver::plan_major(int next, int t, int skip_first) {
+  const int K = P_.refactor_freq;
+  int L = P_.candidates;
+  L = std::min(L, K - (next % K));
+  if (P_.max_iterations > 0) L = std::min(L, P_.max_iterations - next + 1);
+  L = std::min(L, eta_cap_ - 1 - t);
+  if (!marked_ && P_.mark_iteration > 0 && next <= P_.mark_iteration) L = std::min(L, P_.mark_iteration - next + 1);
+  if (L < 1) L = 1;
+  const DevModel m = M();
+  const DevIter i = I();
+  const DevFactor f = F();
+  prof_event(DS_PART_PRICING);
+  if (!skip_first && next / NEED_RESTART > last_dualcheck_ / NEED_RESTART) {
+    launch_dualcheck(st_, m, i, next);
+    last_dualcheck_ = next;
+  }
+  launch_price_multi(st_, m, i, next, skip_first, P_.perturb, P_.candidates);
+  prof_event(DS_PART_PRICING);
+  for (int c = 0; c < P_.candidates; ++c) launch_btran_cand(st_, m, i, f, t, c);
+  prof_event(DS_PART_BTRAN);
+  for (int k = 0; k < L; ++k) enqueue_minor(next + k, t + k);
+  return L;
+}
+
 // Test hook: the forced event is due at iteration `it` (queued once; queued
 // again if the chunk holding it stopped before reaching it).
 bool Solver::force_due(int event, int it) {
@@ -757,6 +826,7 @@ int Solver::plan_chunk(int phase, int next, int t, int skip_first) {
     if (iteration_ >= force_enq_it_) force_fired_ = true;
     force_enq_it_ = -1;
   }
+  if (phase == 2 && P_.candidates > 1) return plan_major(next, t, skip_first);
   int L = 0;
   for (int i = next;; ++i) {
     if (L > 0) {
@@ -973,6 +1043,8 @@ bool Solver::phase2() {
         status_ = DS_OPTIMAL;
         message_ = "optimal solution";
         return true;
+      case ST_MAJOR_END:
+        break;  // no eligible candidate left: the next major iteration prices again
       case ST_NO_PIVOT:
         restarts_++;
         prof_host_begin();

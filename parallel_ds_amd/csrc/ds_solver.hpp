@@ -121,6 +121,12 @@ class Solver {
   DBuf<int> blist_, flist_;
   DBuf<double> bcc_, dxb_;
   bool has_boxed_ = false;
+  // multiple candidate rows (params.candidates > 1)
+  DBuf<int> cand_pos_, cand_act_;
+  DBuf<double> cand_w_, rho_c_;
+  int last_dualcheck_ = 0;
+  void enqueue_minor(int it, int t);
+  int plan_major(int next, int t, int skip_first);
   DBuf<DevScalars> S_;
   DevScalars* hS_ = nullptr;  // pinned mirror
   // factor

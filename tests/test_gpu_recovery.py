@@ -134,3 +134,35 @@ def test_multiround_factor_without_dependency(gpu):
             e[p] = 1.0
             ref = np.linalg.solve(B.T, e)
             assert np.abs(rho - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("cands", [2, 4, 8])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_multiple_candidate_rows_reach_the_optimum(gpu, cands, seed):
+    """Several leaving-row candidates per major iteration (the reference's parallel dual
+    simplex, on one GPU): a different pivot sequence, the same optimum (HiGHS) and status."""
+    L = lpmod.config0(seed)
+    ref = HI["config0"][seed]["objective"]
+    sol0, b0, r0 = solve(L)
+    sol, basis, r = solve(L, candidates=cands)
+    print("candidates %d: %d iterations (sequential %d)" % (cands, r.iterations, r0.iterations))
+    assert r.status == DS_OPTIMAL
+    assert abs(sol.Z - ref) <= 1e-9 * abs(ref)
+    pinf, dinf, _ = lpcheck.certificate(L, sol.x, basis)
+    assert pinf <= 1e-7 and dinf <= 1e-7
+
+
+@pytest.mark.parametrize("cands", [4])
+def test_multiple_candidates_on_boxed_and_scaled_single_lp(gpu, cands):
+    import mixed_lps
+    for s in range(2):
+        L = lpmod.boxed_instance(200, 400, 0.05, s)
+        hs, hz = mixed_lps.highs(L)
+        sol, basis, r = solve(L, candidates=cands)
+        assert r.status == DS_OPTIMAL and abs(sol.Z - hz) <= 1e-9 * abs(hz)
+    g = HI["kcol_m2000_n4000_k10_seed1"]
+    L = lpmod.gen_kcol(2000, 4000, 10, 1)
+    sol, basis, r = solve(L, candidates=cands, refactor_freq=64, max_iterations=400000)
+    print("scaled single LP, %d candidates: %d iterations, %.0f ms" % (cands, r.iterations, r.solve_ms))
+    assert r.status == DS_OPTIMAL and abs(sol.Z - g["objective"]) <= 1e-9 * abs(g["objective"])
+    assert sorted(basis.tolist()) == sorted(g["basis"])
