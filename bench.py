@@ -39,9 +39,9 @@ METRIC = "solves/s + ms/iter, learned-warm vs slack-start on same GPU, obj rel e
 # 262,144 KiB read: x 1.99976; WRITE_SIZE reads the copy's writes exactly: x 1.0).  Counters
 # cannot be read inside a timed run, so other workloads report null.
 #   single LP m=20,000 n=40,000, 10 nnz/col, seed 1, refactor every 64 (iterations 1..300):
-#     3,635.3 KiB x 1.99976 + 316.7 KiB
+#     3,633.3 KiB x 1.99976 + 316.7 KiB
 #   transportation LP 500 x 1000, seed 2 (all 2,294 iterations): 9,916.4 KiB x 1.99976 + 3,849.1 KiB
-SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7768498}
+SPMV_TRAFFIC_BYTES = {(20000, 40000, 10, 1, 64): 7764450}
 TRANSPORT_SPMV_TRAFFIC_BYTES = {(500, 1000, 2): 24247818}
 #   4,096 batched packing LPs (seeds 3000+k), the timed k_batch<512> launch of 479 ms:
 #     584,897,945 KiB x 1.99976 + 906,623,171 KiB
@@ -609,6 +609,7 @@ def main():
     ap.add_argument("--no-transport", action="store_true")
     ap.add_argument("--no-spmv-sweep", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-family", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy a device buffer of this size once (the reference kernel "
@@ -620,8 +621,12 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
-    fam = learned_warm_c4(args.steps, args.warmup, args.train, args.threads, args.family_refactor, args.epochs,
-                          0 if args.no_cpu_baseline else args.family_cpu_iters)
+    if args.no_family:  # profiling runs of the other legs only: no headline value
+        fam = {"learned_warm": {"solves_per_s": None, "wall_s": 0.0}, "slack": {"solves_per_s": None},
+               "speedup_solves_per_s": None, "workload": "not run (--no-family)"}
+    else:
+        fam = learned_warm_c4(args.steps, args.warmup, args.train, args.threads, args.family_refactor, args.epochs,
+                              0 if args.no_cpu_baseline else args.family_cpu_iters)
     single = None if args.no_single else single_lp_window(args)
     out = {
         "metric": METRIC,
