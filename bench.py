@@ -382,7 +382,7 @@ def _solve_many(items, prm_of, threads):
         t = time.perf_counter()
         sol, basis, r = solve_lp(lp, prm, start=start)
         return (time.perf_counter() - t, r.status, sol.Z, r.iterations, r.repairs, basis, r.solve_ms,
-                r.phase1_iterations, r.restarts)
+                r.phase1_iterations, r.restarts, r.primal_iterations, r.shifted_bounds)
 
     t0 = time.perf_counter()
     out = [None] * len(items)
@@ -396,7 +396,7 @@ def _solve_many(items, prm_of, threads):
 
 
 def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_iters, cpu_from=1500, m=2000,
-                    n=5000, k=5, seed=4, sigma=0.1, slack_limit_s=30.0):
+                    n=5000, k=5, seed=4, sigma=0.1, slack_limit_s=30.0, warm_primal=1):
     """Learned warm start against slack start at the family's stated size: the single-LP recipe at
     m=2,000, n=5,000 with 5 nonzeros per column (base seed 4), b and c times lognormal(0, 0.1).
     The network is trained on the optimal bases the GPU solver finds for n_train instances (seeds
@@ -423,7 +423,7 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
     train = [train[i] for i in keep]
     bases = [tr[i][5] for i in keep]
     limit_ms = 2.0 * float(np.mean([r[6] for r in tr_ok]))
-    warm_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=limit_ms)
+    warm_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=limit_ms, warm_primal=warm_primal)
     with BatchSolver(train) as T, Mlp(seed=1) as net:
         T.set_bases(bases)
         losses, fit_ms = net.fit(T, epochs=epochs, minibatch=8192, lr=1e-3, seed=3)
@@ -474,6 +474,9 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
                          "mean_iterations": round(float(np.mean([r[3] for r in rw])), 1),
                          "mean_phase1_iterations": round(float(np.mean([r[7] for r in rw])), 1),
                          "mean_restarts": round(float(np.mean([r[8] for r in rw])), 1),
+                         "primal_phase": bool(warm_primal),
+                         "mean_primal_iterations": round(float(np.mean([r[9] for r in rw])), 1),
+                         "mean_shifted_bounds": round(float(np.mean([r[10] for r in rw])), 1),
                          "fallbacks_to_slack": fallbacks, "time_limit_ms": round(limit_ms, 1),
                          "repaired_columns_mean": round(float(np.mean([r[4] for r in rw])), 2),
                          "predict_s": round(pred_s, 4)},
@@ -588,6 +591,8 @@ def main():
     ap.add_argument("--family-refactor", type=int, default=150)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--family-cpu-iters", type=int, default=300)
+    ap.add_argument("--warm-primal", type=int, default=1,
+                    help="1: primal simplex iterations from the bound-shifted predicted basis before the dual simplex")
     ap.add_argument("--m", type=int, default=20000)
     ap.add_argument("--n", type=int, default=40000)
     ap.add_argument("--k", type=int, default=10)
@@ -626,7 +631,7 @@ def main():
                "speedup_solves_per_s": None, "workload": "not run (--no-family)"}
     else:
         fam = learned_warm_c4(args.steps, args.warmup, args.train, args.threads, args.family_refactor, args.epochs,
-                              0 if args.no_cpu_baseline else args.family_cpu_iters)
+                              0 if args.no_cpu_baseline else args.family_cpu_iters, warm_primal=args.warm_primal)
     single = None if args.no_single else single_lp_window(args)
     out = {
         "metric": METRIC,

@@ -91,6 +91,16 @@ typedef struct {
                              its own, and the others' rows follow each basis change */
   double candidate_psi;   /* a candidate drops out when its weight falls below psi times its previous
                              one (the reference's psi, main.cpp:46-53); default 0.9 */
+  int warm_primal;        /* starts from ds_init_basis only, no counterpart in the reference: 1 = before
+                             the dual simplex, make the starting basis primal feasible by moving each
+                             violated bound of a basic variable warm_shift * (1 + |x_j|) past its value,
+                             run primal simplex iterations (Devex pricing, two-pass ratio test) until no
+                             reduced cost is dual infeasible, restore the bounds, and let the dual
+                             simplex (phase 1 if needed, then phase 2) finish from that dual feasible
+                             basis.  Models with Free or Boxed variables skip it.  0 (default): the dual
+                             simplex starts from the given basis directly */
+  double warm_shift;      /* relative bound shift of warm_primal; default 1e-3 */
+  int warm_primal_cap;    /* most primal iterations before the dual simplex takes over; 0 = 4 m */
 } ds_params;
 
 /* Recovery paths a test can force from the host (sequentialDualSimplex.cpp):
@@ -135,6 +145,8 @@ typedef struct {
   int cleanups;           /* dual simplex resumptions after restoring perturbed costs */
   int repairs;            /* structural columns replaced by slacks in singular factorizations */
   double step_ms[DS_NPARTS]; /* with params.profile: time per step (DS_PART_*) */
+  int primal_iterations;  /* params.warm_primal: primal simplex iterations (part of `iterations`) */
+  int shifted_bounds;     /* params.warm_primal: bounds of basic variables moved for primal feasibility */
 } ds_result;
 
 void ds_default_params(ds_params* p);
@@ -306,6 +318,11 @@ int ds_mlp_set_params(ds_mlp* net, const float* params);
 /* Feature standardisation (x - mu) / sd applied before the first layer. */
 int ds_mlp_set_norm(ds_mlp* net, const float* mu, const float* sd);
 int ds_mlp_get_norm(ds_mlp* net, float* mu, float* sd);
+/* The family prior the last ds_mlp_fit found (feature 16: per variable index,
+ * the fraction of the training bases in which it is basic; set only when every
+ * training LP has the same m and n).  *n receives its length (0 = none); the
+ * first min(cap, *n) values go to out. */
+int ds_mlp_get_prior(ds_mlp* net, float* out, int cap, int* n);
 /* Logits of S feature rows (S x n_features, host buffers). */
 int ds_mlp_infer(ds_mlp* net, int S, const float* features, float* logits);
 /* Mean weighted BCE-with-logits loss of S rows and its gradient w.r.t. the
@@ -322,7 +339,8 @@ int ds_mlp_fit(ds_mlp* net, ds_ctx* batch, int epochs, int minibatch, float lr, 
 /* Device time (ms) and MFMA flops of the training GEMMs so far. */
 int ds_mlp_train_stats(ds_mlp* net, double* ms, long long* flops);
 
-/* Per-variable features (sum(n_k+m_k) x 16, FP32) and 0/1 labels of the last
+/* Per-variable features (sum(n_k+m_k) x 17, FP32; column 16, the family prior,
+ * is 0 here and filled only when a trained network predicts) and 0/1 labels of the last
  * solved bases (sum(n_k+m_k)), host copies. */
 int ds_batch_features(ds_ctx* ctx, float* features);
 int ds_batch_labels(ds_ctx* ctx, float* labels);

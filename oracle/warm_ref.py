@@ -24,7 +24,7 @@ import numpy as np
 
 WIDTH = 256
 KPAD = 32
-NF = 16
+NF = 17
 
 
 def layout(width=WIDTH, kpad=KPAD):
@@ -146,8 +146,9 @@ def adam(p, g, m1, m2, t, lr, b1=0.9, b2=0.999, eps=1e-8):
 
 
 def features(m, n, col_ptr, row_idx, val, b, c):
-    """(n+m) x 16 features of the packing LP min c^T x, A x <= b, x >= 0
-    (structurals first, then the slacks), in FP64."""
+    """(n+m) x 17 features of the packing LP min c^T x, A x <= b, x >= 0
+    (structurals first, then the slacks), in FP64.  Column 16, the family prior,
+    is 0 here: only a trained network's prediction step fills it (family_prior)."""
     col_ptr = np.asarray(col_ptr)
     row_idx = np.asarray(row_idx)
     val = np.asarray(val, np.float64)
@@ -191,6 +192,17 @@ def features(m, n, col_ptr, row_idx, val, b, c):
     F[n:, 10] = rank(t) / m
     F[n:, 11] = rank(y) / m
     return F
+
+
+def family_prior(labels, loo):
+    """Column 16 of the learned warm start's features for a family of T LPs of the
+    same shape: labels is T x N (1 = basic in that LP's training basis); the
+    fraction of the training bases in which each variable is basic, over the
+    other T - 1 LPs for a training row (loo=True), over all T for a prediction."""
+    Y = np.asarray(labels, np.float64)
+    T = Y.shape[0]
+    cnt = Y.sum(0)
+    return (cnt[None, :] - Y) / (T - 1) if loo else cnt / T
 
 
 def rank(v):

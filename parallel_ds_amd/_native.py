@@ -20,7 +20,8 @@ class Params(C.Structure):
                 ("perturb", C.c_int), ("perturb_seed", C.c_uint64), ("chunk", C.c_int), ("verbose", C.c_int),
                 ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
                 ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int),
-                ("time_limit_ms", C.c_double), ("candidates", C.c_int), ("candidate_psi", C.c_double)]
+                ("time_limit_ms", C.c_double), ("candidates", C.c_int), ("candidate_psi", C.c_double),
+                ("warm_primal", C.c_int), ("warm_shift", C.c_double), ("warm_primal_cap", C.c_int)]
 
 
 DS_FORCE_NONE, DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART, DS_FORCE_SINGULAR = 0, 1, 2, 3
@@ -37,7 +38,8 @@ class Result(C.Structure):
                 ("restarts", C.c_int), ("Z", C.c_double), ("solve_ms", C.c_double), ("refactor_ms", C.c_double),
                 ("mark_ms", C.c_double), ("spmv_ms", C.c_double), ("spmv_launches", C.c_int), ("kernel_k", C.c_int),
                 ("nnz_nonbasic_mark", C.c_double), ("nnz_nonbasic_end", C.c_double), ("cleanups", C.c_int),
-                ("repairs", C.c_int), ("step_ms", C.c_double * 13)]
+                ("repairs", C.c_int), ("step_ms", C.c_double * 13), ("primal_iterations", C.c_int),
+                ("shifted_bounds", C.c_int)]
 
 
 class BatchParams(C.Structure):
@@ -111,6 +113,7 @@ _SIGS = {
     "ds_mlp_set_params": (I32, [VP, VP]),
     "ds_mlp_set_norm": (I32, [VP, VP, VP]),
     "ds_mlp_get_norm": (I32, [VP, VP, VP]),
+    "ds_mlp_get_prior": (I32, [VP, VP, I32, C.POINTER(I32)]),
     "ds_mlp_infer": (I32, [VP, I32, VP, VP]),
     "ds_mlp_grad": (I32, [VP, I32, VP, VP, VP, C.POINTER(F64), VP]),
     "ds_mlp_adam_step": (I32, [VP, C.c_float]),

@@ -114,8 +114,9 @@ class BatchSolver:
         return int(self.voff[-1])
 
     def features(self):
-        """Per-variable features, (sum(n_k+m_k), 16) FP32, computed on the device."""
-        F = np.zeros((self.nvars, 16), np.float32)
+        """Per-variable features, (sum(n_k+m_k), 17) FP32, computed on the device (column 16, the
+        family prior, is filled only when a trained network predicts)."""
+        F = np.zeros((self.nvars, 17), np.float32)
         self._check(lib().ds_batch_features(self._ctx, ptr(F)), "ds_batch_features")
         return F
 

@@ -107,6 +107,9 @@ void ds_default_params(ds_params* p) {
   p->time_limit_ms = 0.0;
   p->candidates = 1;
   p->candidate_psi = 0.9;
+  p->warm_primal = 0;
+  p->warm_shift = 1e-3;
+  p->warm_primal_cap = 0;
 }
 
 int ds_create(int device, ds_ctx** out) {
@@ -481,6 +484,15 @@ int ds_mlp_set_norm(ds_mlp* net, const float* mu, const float* sd) {
 int ds_mlp_get_norm(ds_mlp* net, float* mu, float* sd) {
   if (!mu || !sd) return DS_EINVAL;
   return mguard(net, [&](pds::Mlp& M) { M.get_norm(mu, sd); });
+}
+
+int ds_mlp_get_prior(ds_mlp* net, float* out, int cap, int* n) {
+  if (!n || cap < 0 || (cap > 0 && !out)) return DS_EINVAL;
+  return mguard(net, [&](pds::Mlp& M) {
+    const std::vector<float>& p = M.prior();
+    *n = (int)p.size();
+    for (int j = 0; j < cap && j < (int)p.size(); ++j) out[j] = p[j];
+  });
 }
 
 int ds_mlp_infer(ds_mlp* net, int S, const float* features, float* logits) {

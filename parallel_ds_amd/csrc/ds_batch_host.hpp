@@ -29,7 +29,9 @@ class BatchSolver {
   const int* dev_m() const { return m_.p; }
   const int* dev_n() const { return n_.p; }
   const long long* dev_vo() const { return vo_.p; }
-  // Learned warm start: per-variable features (vars() x 16, FP32), 0/1
+  const std::vector<int>& host_m() const { return hm_; }
+  const std::vector<int>& host_n() const { return hn_; }
+  // Learned warm start: per-variable features (vars() x WS_NF, FP32), 0/1
   // labels of the last solve's bases, starting bases from per-variable scores
   // (top m per LP), and the pivot tolerance below which their first
   // factorization substitutes slacks.  Device pointers, stream-ordered.

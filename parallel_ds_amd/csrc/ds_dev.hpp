@@ -56,6 +56,8 @@ struct DevIter {
   double* cand_w;          // their last weights delta^2 / beta
   int* cand_act;           // still eligible
   double* rho_c;           // CAND_MAX x m: their BTRAN rows, kept current through the minor iterations
+  // Primal simplex iterations of a warm start (params.warm_primal):
+  double* devex;           // N: Devex reference weights by variable
   int eta_cap;
   unsigned char* blocked;  // N
   int* blocked_list;       // MAX_BLOCKED

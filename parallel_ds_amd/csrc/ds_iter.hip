@@ -932,7 +932,8 @@ __device__ __forceinline__ UpdateArgs update_args(const DevModel& M, const DevIt
 }
 
 // Grid-stride part of the basis change: x_B (or f), the new eta, DSE weights.
-__device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u) {
+__device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u,
+                                            bool dse = true) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
     double ai = I.alpha_q[i];
     if (phase == 2) {
@@ -944,7 +945,7 @@ __device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I,
     if (i != u.p_idx) {
       u.eta[i] = -ai / u.ap;
       double r = ai / u.ap;
-      I.beta[i] = I.beta[i] - 2 * ai / u.ap * I.tau[i] + (r * r) * u.bp;
+      if (dse) I.beta[i] = I.beta[i] - 2 * ai / u.ap * I.tau[i] + (r * r) * u.bp;
     } else {
       u.eta[i] = 1 / u.ap;
     }
@@ -1046,6 +1047,197 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
     dupdate_final(M, I, phase, it, theta, td);
   }
   if (stable) update_final(M, I, phase, u);
+}
+
+// ---------------------------------------------------------------------------
+// Primal simplex iterations of a warm start (params.warm_primal; no
+// counterpart in the reference).  The host makes the starting basis primal
+// feasible by moving each violated bound of a basic variable just past its
+// value; these iterations then remove the dual infeasibilities: Devex pricing
+// over the nonbasic set, FTRAN of the entering column, a two-pass (Harris)
+// ratio test over the basic rows, BTRAN of the leaving row, the pivot-row SpMV
+// and the update of d, x_B and the eta file.  With d feasible the host
+// restores the bounds and the dual simplex finishes from that basis.  One
+// iteration: k_pprice -> FTRAN -> k_pratio1/2 -> BTRAN -> SpMV -> k_pprep ->
+// k_pupdate.  Models with Free or Boxed variables do not take this path.
+// ---------------------------------------------------------------------------
+constexpr double PRIMAL_PIV = 1e-7;  // smallest |alpha_q[i]| of a leaving row
+constexpr double PRIMAL_TOL = 1e-9;  // feasibility slack of the first ratio pass
+
+// Entering column: the largest d_j^2 / w_j over nonbasic Lower variables with
+// d_j < -EPS_D and Upper ones with d_j > EPS_D (ties to the lowest position).
+// None left: ST_OPTIMAL (the basis is dual feasible for the true costs).
+// S->neg records the direction: 0 = x_q increases, 1 = it decreases.
+__global__ __launch_bounds__(BS) void k_pprice(DevModel M, DevIter I, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const int nN = M.N - M.m;
+  double bw = -1.0;
+  int bi = INT_MAX;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    const int j = I.nonbasis[i];
+    const int8_t t = M.bt[j];
+    const double dj = I.d[j];
+    if ((t == BT_LOWER && dj < -EPS_D) || (t == BT_UPPER && dj > EPS_D))
+      argmax_merge(bw, bi, dj * dj / I.devex[j], i);
+  }
+  block_argmax<BS>(bw, bi);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = bw;
+    I.part_i[blockIdx.x] = bi;
+  }
+  if (!last_block(&S->ctr[CTR_PRICE])) return;
+  double fw = -1.0;
+  int fi = INT_MAX;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fw, fi, I.part_d[b], I.part_i[b]);
+  block_argmax<BS>(fw, fi);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_PRICE] = 0;
+  if (fi == INT_MAX) {
+    raise_status(S, ST_OPTIMAL, it);
+    return;
+  }
+  const int q = I.nonbasis[fi];
+  S->q_idx = fi;
+  S->q = q;
+  S->neg = I.d[q] < 0 ? 0 : 1;
+}
+
+// Ratio of basic row i for the step of x_q in direction dir: the row blocks
+// at its lower bound when a = dir * alpha_q[i] > PRIMAL_PIV, at its upper
+// bound when a < -PRIMAL_PIV (x_B moves by -step * alpha_q).  Returns false
+// when the row does not block; bound receives the blocking bound.
+__device__ __forceinline__ bool primal_row(const DevModel& M, const DevIter& I, double dir, int i, double& a,
+                                           double& x, double& bound) {
+  a = dir * I.alpha_q[i];
+  const int j = I.basis[i];
+  const int8_t t = M.bt[j];
+  x = I.x[j];
+  if (a > PRIMAL_PIV && t != BT_UPPER && t != BT_FREE) {
+    bound = M.l[j];
+    return true;
+  }
+  if (a < -PRIMAL_PIV && (t == BT_UPPER || t == BT_FIXED || t == BT_BOXED)) {
+    bound = M.u[j];
+    return true;
+  }
+  return false;
+}
+
+// Pass 1: theta_max = min over blocking rows of the ratio with the bound
+// relaxed by PRIMAL_TOL (at least 0).  No blocking row: ST_NO_PIVOT (the
+// direction is unbounded for the shifted bounds; the host gives up the
+// primal iterations).
+__global__ __launch_bounds__(BS) void k_pratio1(DevModel M, DevIter I, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double dir = S->neg ? -1.0 : 1.0;
+  double mn = BIG;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    double a, x, bd;
+    if (!primal_row(M, I, dir, i, a, x, bd)) continue;
+    const double r = a > 0 ? (x - bd + PRIMAL_TOL) / a : (x - bd - PRIMAL_TOL) / a;
+    mn = fmin(mn, fmax(r, 0.0));
+  }
+  mn = block_min<BS>(mn);
+  if (threadIdx.x == 0) I.part_d[blockIdx.x] = mn;
+  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  double g = BIG;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) g = fmin(g, I.part_d[b]);
+  g = block_min<BS>(g);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_CHUZC1] = 0;
+  S->tmp0 = g;
+  if (!(g < BIG)) raise_status(S, ST_NO_PIVOT, it);
+}
+
+// Pass 2: among blocking rows whose exact ratio is within theta_max, the
+// largest |alpha_q[i]| (ties to the lowest position) leaves; delta = x_p -
+// its bound, so the basis change moves x_q by delta / alpha_q[p].
+__global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double dir = S->neg ? -1.0 : 1.0;
+  const double g = S->tmp0;
+  double best = -1.0;
+  int bi = INT_MAX;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    double a, x, bd;
+    if (!primal_row(M, I, dir, i, a, x, bd)) continue;
+    if ((x - bd) / a <= g) argmax_merge(best, bi, fabs(a), i);
+  }
+  block_argmax<BS>(best, bi);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = best;
+    I.part_i[blockIdx.x] = bi;
+  }
+  if (!last_block(&S->ctr[CTR_CHUZC2])) return;
+  double fb = -1.0;
+  int fi = INT_MAX;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fb, fi, I.part_d[b], I.part_i[b]);
+  block_argmax<BS>(fb, fi);
+  if (threadIdx.x != 0) return;
+  S->ctr[CTR_CHUZC2] = 0;
+  if (fi == INT_MAX) {
+    raise_status(S, ST_NO_PIVOT, it);
+    return;
+  }
+  double a, x, bd;
+  primal_row(M, I, dir, fi, a, x, bd);
+  S->p_idx = fi;
+  S->p = I.basis[fi];
+  S->delta = x - bd;
+  S->is_lower = a > 0 ? 1 : 0;
+}
+
+// After the pivot-row SpMV: alpha_r[q] for the stability check, the dual step
+// theta = d_q / alpha_q[p] (alpha_q[p] passed the pivot tolerance) and the
+// entering column's Devex weight.
+__global__ void k_pprep(DevModel M, DevIter I) {
+  DevScalars* S = I.S;
+  if (stopped(S) || threadIdx.x != 0) return;
+  S->alpha_rq = I.alpha_r[S->q_idx];
+  S->theta = I.d[S->q] / I.alpha_q[S->p_idx];
+  S->tmp1 = I.devex[S->q];
+}
+
+// d_N -= theta alpha_r with the Devex weights w_j = max(w_j, (alpha_j /
+// alpha_q)^2 w_q), then the basis change of k_dupdate_update without the DSE
+// weights (they are recomputed exactly when the dual simplex takes over).
+__global__ __launch_bounds__(BS) void k_pupdate(DevModel M, DevIter I, int it) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double theta = S->theta;
+  const UpdateArgs u = update_args(M, I, 2);
+  const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
+  const double arq = S->alpha_rq, wq = S->tmp1;
+  const int q_idx = S->q_idx;
+  const int nN = M.N - M.m;
+  int dinf = 0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+    const int j = I.nonbasis[i];
+    const double a = I.alpha_r[i];
+    const double dj = I.d[j] - theta * a;
+    I.d[j] = dj;
+    if (i != q_idx) {
+      dinf += dual_infeas(M.bt[j], dj);
+      const double r = a / arq;
+      I.devex[j] = fmax(I.devex[j], r * r * wq);
+    }
+  }
+  if (stable) update_body(M, I, 2, u, false);
+  dinf = block_sumi<BS>(dinf);
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = dinf;
+  if (!last_block(&S->ctr[CTR_DUPD])) return;
+  int td = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) td += I.part_i[b];
+  td = block_sumi<BS>(td);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_DUPD] = 0;
+    dupdate_final(M, I, 2, it, theta, td);
+    I.devex[S->p] = fmax(wq / (arq * arq), 1.0);
+  }
+  if (stable) update_final(M, I, 2, u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1714,6 +1906,19 @@ void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   k_dupdate_update<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
+}
+void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
+  k_pprice<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
+}
+void launch_pratio(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
+  const int g = cap_grid(M.m, BS, I.part_cap);
+  k_pratio1<<<g, BS, 0, s>>>(M, I, it);
+  k_pratio2<<<g, BS, 0, s>>>(M, I, it);
+}
+void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
+  k_pprep<<<1, WAVE, 0, s>>>(M, I);
+  const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
+  k_pupdate<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, it);
 }
 void launch_bfrt(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int harris, int t) {
   k_bfrt<<<1, BFRT_BS, 0, s>>>(M, I, harris);

@@ -24,6 +24,12 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
 void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
+// Primal simplex iterations of a warm start: entering column (Devex), the
+// two-pass ratio test over the basic rows, and the update (after FTRAN, BTRAN
+// and the pivot-row SpMV).
+void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it);
+void launch_pratio(hipStream_t s, const DevModel& M, const DevIter& I, int it);
+void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it);
 // Bound-flipping ratio test: the flip loop after CHUZC, then the flip set, the
 // column change, its FTRAN and the x_B / objective correction.
 void launch_bfrt(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int harris, int t);

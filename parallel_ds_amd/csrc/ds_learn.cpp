@@ -16,9 +16,31 @@ double now_ms() {
 }
 }  // namespace
 
+// Family prior (feature 16).  When every LP of the batch has the same m and n
+// (instances of one family, sharing A), feature 16 of variable j is the
+// fraction of the training bases in which j is basic.  A training row sees the
+// fraction over the other training instances (leave one out), so the network
+// learns how far to trust it; a prediction sees the fraction over all of them.
+static bool family_shape(const BatchSolver& B, int& N) {
+  const std::vector<int>& m = B.host_m();
+  const std::vector<int>& n = B.host_n();
+  if (m.empty()) return false;
+  for (size_t k = 1; k < m.size(); ++k)
+    if (m[k] != m[0] || n[k] != n[0]) return false;
+  N = m[0] + n[0];
+  return true;
+}
+
+// Writes column 16 of the device feature matrix F (R rows) from a host column.
+static void put_prior_column(hipStream_t st, float* F, const std::vector<float>& col) {
+  PDS_HIP(hipMemcpy2DAsync(F + (WS_NF - 1), sizeof(float) * WS_NF, col.data(), sizeof(float), sizeof(float),
+                           col.size(), hipMemcpyHostToDevice, st));
+  PDS_HIP(hipStreamSynchronize(st));
+}
+
 double fit_from_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, int epochs, int minibatch, float lr,
                       unsigned long long seed, double* epoch_loss) {
-  if (net.n_features() != WS_NF) throw std::invalid_argument("network must take the 16 batch features");
+  if (net.n_features() != WS_NF) throw std::invalid_argument("network must take the batch features (17)");
   if (epochs <= 0 || minibatch <= 0) throw std::invalid_argument("epochs and minibatch must be positive");
   const long long R = B.vars();
   if (R <= 0 || R > 0x7fffffffLL) throw std::invalid_argument("training set size out of range");
@@ -28,6 +50,23 @@ double fit_from_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, int epochs, int
   wb.sd.alloc(WS_NF);
   B.features(wb.F.p);
   B.labels(wb.y.p);
+  int N = 0;
+  std::vector<float> prior;
+  if (B.count() > 1 && family_shape(B, N)) {
+    const int T = B.count();
+    std::vector<float> y(R), col(R);
+    PDS_HIP(hipMemcpyAsync(y.data(), wb.y.p, sizeof(float) * R, hipMemcpyDeviceToHost, B.stream()));
+    PDS_HIP(hipStreamSynchronize(B.stream()));
+    std::vector<double> cnt(N, 0.0);
+    for (int k = 0; k < T; ++k)
+      for (int j = 0; j < N; ++j) cnt[j] += y[(size_t)k * N + j];
+    for (int k = 0; k < T; ++k)
+      for (int j = 0; j < N; ++j) col[(size_t)k * N + j] = (float)((cnt[j] - y[(size_t)k * N + j]) / (T - 1));
+    put_prior_column(B.stream(), wb.F.p, col);
+    prior.resize(N);
+    for (int j = 0; j < N; ++j) prior[j] = (float)(cnt[j] / T);
+  }
+  net.set_prior(prior);
   launch_colstats(B.stream(), wb.F.p, R, wb.mu.p, wb.sd.p);
   PDS_HIP(hipGetLastError());
   std::vector<float> mu(WS_NF), sd(WS_NF);
@@ -54,13 +93,20 @@ double fit_from_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, int epochs, int
 }
 
 double warm_start_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, double* infer_ms) {
-  if (net.n_features() != WS_NF) throw std::invalid_argument("network must take the 16 batch features");
+  if (net.n_features() != WS_NF) throw std::invalid_argument("network must take the batch features (17)");
   const long long R = B.vars();
   if (R <= 0 || R > 0x7fffffffLL) throw std::invalid_argument("batch size out of range");
   wb.F.alloc((size_t)R * WS_NF);
   wb.z.alloc(R);
   const double t0 = now_ms();
   B.features(wb.F.p);
+  int N = 0;
+  const std::vector<float>& pr = net.prior();
+  if (!pr.empty() && family_shape(B, N) && N == (int)pr.size()) {
+    std::vector<float> col(R);
+    for (long long r = 0; r < R; ++r) col[r] = pr[r % N];
+    put_prior_column(B.stream(), wb.F.p, col);
+  }
   PDS_HIP(hipStreamSynchronize(B.stream()));
   net.infer(wb.F.p, nullptr, (int)R, wb.z.p);  // synchronises its stream
   B.select_warm(wb.z.p);

@@ -66,6 +66,10 @@ class Mlp {
   double last_infer_ms() const { return infer_ms_; }
   double gemm_ms() const { return gemm_ms_; }
   long long gemm_flops() const { return gemm_flops_; }
+  // Family prior of the learned warm start (feature 16): per variable index,
+  // the fraction of the training bases in which it is basic; empty = none.
+  void set_prior(std::vector<float> p) { prior_ = std::move(p); }
+  const std::vector<float>& prior() const { return prior_; }
   void reset_counters() {
     gemm_ms_ = 0.0;
     gemm_flops_ = 0;
@@ -83,6 +87,7 @@ class Mlp {
   double infer_ms_ = 0.0, gemm_ms_ = 0.0;
   long long gemm_flops_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  std::vector<float> prior_;
   void refresh();
   void reserve(int S);
 };

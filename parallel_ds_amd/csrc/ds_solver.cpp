@@ -218,6 +218,7 @@ DevIter Solver::I() const {
   it.alpha_r = alpha_r_.p;
   it.alpha_q = alpha_q_.p;
   it.tau = tau_.p;
+  it.devex = devex_.p;
   it.w1 = w1_.p;
   it.w2 = w2_.p;
   it.wN = wN_.p;
@@ -377,7 +378,12 @@ void Solver::load(const HostModel& hm) {
   nA_.alloc(m / 256 + 2);
   nB_.alloc(m / 256 + 2);
   has_boxed_ = false;
-  for (int j = 0; j < N; ++j) has_boxed_ = has_boxed_ || H_.bt[j] == BT_BOXED;
+  primal_ok_ = true;
+  for (int j = 0; j < N; ++j) {
+    has_boxed_ = has_boxed_ || H_.bt[j] == BT_BOXED;
+    primal_ok_ = primal_ok_ && H_.bt[j] != BT_BOXED && H_.bt[j] != BT_FREE;
+  }
+  devex_.alloc(N);
   const size_t nb = has_boxed_ ? (size_t)n : 1, mb = has_boxed_ ? (size_t)m : 1;
   bmark_.alloc(nb);
   blist_.alloc(nb);
@@ -683,6 +689,7 @@ void Solver::initDualSimplex() {
   iteration_ = 0;
   t_ = 0;
   refactors_ = restarts_ = phase1_iters_ = cleanups_ = repairs_ = 0;
+  primal_iters_ = shifted_ = 0;
   warm_ = false;
   repaired_ = false;
   repair_ = false;
@@ -1119,6 +1126,141 @@ bool Solver::phase2() {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Primal simplex iterations of a warm start (params.warm_primal; the reference
+// always starts from the slack basis).  Bounds of basic variables that the
+// starting basis violates are moved warm_shift * (1 + |x_j|) past x_j, which
+// makes the basis primal feasible; primal iterations (Devex pricing, two-pass
+// ratio test) then run until no reduced cost is dual infeasible.  The bounds
+// are restored and the basis refactored on the way out: the dual simplex
+// continues from a basis that is dual feasible for the true costs, and its
+// phase 2 removes the primal infeasibilities the restored bounds leave.
+// ---------------------------------------------------------------------------
+// Moves the violated bounds of the current basic variables (on the host copies
+// lw_/uw_, then uploaded); returns how many moved.
+int Solver::shift_bounds() {
+  const int m = H_.m, N = H_.N;
+  std::vector<double> x(N);
+  PDS_HIP(hipMemcpyAsync(x.data(), x_.p, sizeof(double) * N, hipMemcpyDeviceToHost, st_));
+  std::vector<int> b = basis_host();
+  int moved = 0;
+  for (int i = 0; i < m; ++i) {
+    const int j = b[i];
+    const double xj = x[j], gap = P_.warm_shift * (1.0 + std::fabs(xj));
+    const int8_t t = H_.bt[j];
+    if ((t == BT_LOWER || t == BT_FIXED) && xj < lw_[j]) {
+      lw_[j] = xj - gap;
+      ++moved;
+    }
+    if ((t == BT_UPPER || t == BT_FIXED) && xj > uw_[j]) {
+      uw_[j] = xj + gap;
+      ++moved;
+    }
+  }
+  if (moved) {
+    PDS_HIP(hipMemcpyAsync(l_.p, lw_.data(), sizeof(double) * N, hipMemcpyHostToDevice, st_));
+    PDS_HIP(hipMemcpyAsync(u_.p, uw_.data(), sizeof(double) * N, hipMemcpyHostToDevice, st_));
+    sync();
+  }
+  return moved;
+}
+
+void Solver::enqueue_primal(int it, int t) {
+  const DevModel m = M();
+  const DevIter i = I();
+  const DevFactor f = F();
+  launch_pprice(st_, m, i, it);
+  launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
+  launch_pratio(st_, m, i, it);
+  launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
+  launch_pupdate(st_, m, i, it);
+}
+
+void Solver::primal_phase() {
+  const int K = P_.refactor_freq;
+  const int cap = P_.warm_primal_cap > 0 ? P_.warm_primal_cap : 4 * H_.m;
+  lw_ = H_.l;
+  uw_ = H_.u;
+  const int it0 = iteration_;
+  // The bounds stay moved only inside this function.
+  struct Restore {
+    Solver* s;
+    ~Restore() {
+      (void)hipMemcpyAsync(s->l_.p, s->H_.l.data(), sizeof(double) * s->H_.N, hipMemcpyHostToDevice, s->st_);
+      (void)hipMemcpyAsync(s->u_.p, s->H_.u.data(), sizeof(double) * s->H_.N, hipMemcpyHostToDevice, s->st_);
+      (void)hipStreamSynchronize(s->st_);
+    }
+  } restore{this};
+  dualSimplexInit();
+  shifted_ += shift_bounds();
+  launch_fill(st_, devex_.p, 1.0, H_.N);
+  pull();
+  hS_->status = ST_RUNNING;
+  hS_->refac_mid = 0;
+  hS_->cycle = 0;
+  push();
+  while (iteration_ - it0 < cap) {
+    const int next = iteration_ + 1;
+    tick_limit(next);
+    if (next % K == 0) {
+      refactor();
+      if (repaired_) {
+        repaired_ = false;
+        restarts_++;
+        dualSimplexInit();
+        shifted_ += shift_bounds();
+      }
+    }
+    int L = 0;
+    for (int i = next;; ++i) {
+      if (L > 0 && (i % K == 0 || L >= P_.chunk || i - it0 > cap)) break;
+      if (P_.max_iterations > 0 && i > P_.max_iterations && L > 0) break;
+      if (t_ + L + 1 >= eta_cap_) break;
+      enqueue_primal(i, t_ + L);
+      ++L;
+    }
+    if (L == 0) {  // eta file full
+      refactor();
+      t_ = 0;
+      continue;
+    }
+    pull();
+    if (hS_->status == ST_RUNNING && !hS_->refac_mid) {
+      iteration_ = next + L - 1;
+      t_ = hS_->t;
+      continue;
+    }
+    const int stop = hS_->stop_iter;
+    t_ = hS_->t;
+    if (hS_->refac_mid && hS_->status == ST_RUNNING) {
+      iteration_ = stop;
+      hS_->refac_mid = 0;
+      push();
+      refactor();
+      if (repaired_) {
+        repaired_ = false;
+        restarts_++;
+        dualSimplexInit();
+        shifted_ += shift_bounds();
+      } else {
+        enqueue_stageB(2, stop, 0);
+      }
+      pull();
+      t_ = hS_->t;
+      continue;
+    }
+    // ST_OPTIMAL: dual feasible; ST_NO_PIVOT: no blocking row, the dual
+    // simplex takes over from this basis either way.
+    iteration_ = stop - 1;
+    hS_->status = ST_RUNNING;
+    hS_->refac_mid = 0;
+    push();
+    break;
+  }
+  primal_iters_ += iteration_ - it0;
+}
+
 // checkDualFeasible (tpp:362-385) on the current reduced costs.
 bool Solver::dual_feasible() {
   pull();
@@ -1156,6 +1298,13 @@ int Solver::presolve() {
   try {
     repair_ = true;
     if (P_.perturb && checkPerturbNeed()) perturbCosts();
+    if (warm_ && P_.warm_primal && primal_ok_) {
+      primal_phase();
+      // A fresh factor of the basis the primal iterations reached; the dual
+      // simplex's iterate is computed from it.
+      refactor();
+      repaired_ = false;
+    }
     initBetaWeights(!warm_);
     st = phase1();
     phase1_iters_ = iteration_;
@@ -1246,6 +1395,8 @@ void Solver::result(ds_result& r) const {
   r.cleanups = cleanups_;
   r.repairs = repairs_;
   for (int k = 0; k < DS_NPARTS; ++k) r.step_ms[k] = step_ms_[k];
+  r.primal_iterations = primal_iters_;
+  r.shifted_bounds = shifted_;
   if (P_.profile) r.step_ms[DS_PART_FACTOR] = refactor_ms_;
 }
 

@@ -121,6 +121,14 @@ class Solver {
   DBuf<int> blist_, flist_;
   DBuf<double> bcc_, dxb_;
   bool has_boxed_ = false;
+  // Primal simplex iterations of a warm start (params.warm_primal).
+  bool primal_ok_ = true;        // no Free or Boxed variables
+  DBuf<double> devex_;
+  std::vector<double> lw_, uw_;  // working bounds with the moved ones
+  int primal_iters_ = 0, shifted_ = 0;
+  int shift_bounds();
+  void enqueue_primal(int it, int t);
+  void primal_phase();
   // multiple candidate rows (params.candidates > 1)
   DBuf<int> cand_pos_, cand_act_;
   DBuf<double> cand_w_, rho_c_;

@@ -22,12 +22,13 @@ constexpr int WBS = 256;
 //  12 rank of feature 10 among the structurals / n
 //  13 rank of d_j among the structurals / n
 //  14 max_i t_i over the rows of a_j     15 max_i y_i / C over those rows
+//  16 family prior (0 here; see ds_learn.cpp)
 // Slack of row i:
 //   0 slack flag (1)      1 0                  2 sum a_i.      3 |a_i.|_2
 //   4 nnz(a_i.)           5 max a_i.           6 b_i / B
 //   7 t_i = sum a_i. / b_i (row load at x = 1)   8 y_i / C
 //   9 sum_j a_ij |c_j| / b_i / C   10 rank of t_i / m   11 rank of y_i / m
-//  12-15 zero
+//  12-15 zero                            16 family prior (0 here)
 // with C = mean |c_j|, B = mean b_i and the row price estimate
 // y_i = mean over the row's columns of |c_j| / sum a_j.
 __global__ __launch_bounds__(WBS) void k_features(BatchDev D, float* __restrict__ F) {
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(WBS) void k_features(BatchDev D, float* __restrict_
     f[11] = dj / Cm;
     f[14] = tmax;
     f[15] = ymax / Cm;
+    f[16] = 0.f;
   }
   __syncthreads();
   // Ranks (ties broken by index), O(n^2) comparisons over the workgroup.
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(WBS) void k_features(BatchDev D, float* __restrict_
     float* f = out + (size_t)(n + i) * WS_NF;
     f[10] = (float)r1 / m;
     f[11] = (float)r2 / m;
-    f[12] = f[13] = f[14] = f[15] = 0.f;
+    f[12] = f[13] = f[14] = f[15] = f[16] = 0.f;
   }
 }
 

@@ -5,7 +5,7 @@
 
 namespace pds {
 
-constexpr int WS_NF = 16;  // features per variable
+constexpr int WS_NF = 17;  // features per variable (the last one is the family prior, filled by the host)
 
 size_t features_lds_bytes(int mmax, int nmax);
 size_t select_lds_bytes(int Nmax);

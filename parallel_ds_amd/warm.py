@@ -6,7 +6,7 @@ import numpy as np
 
 from ._native import DsError, lib, ptr
 
-N_FEATURES = 16
+N_FEATURES = 17  # the 16 LP features of ds_warm.hip and the family prior
 WIDTH = 256
 KPAD = 32
 
@@ -84,6 +84,15 @@ class Mlp:
         sd = np.zeros(self.n_features, np.float32)
         self._check(lib().ds_mlp_get_norm(self._h, ptr(mu), ptr(sd)), "ds_mlp_get_norm")
         return mu, sd
+
+    def prior(self):
+        """Family prior found by the last fit (feature 16), or an empty array."""
+        n = C.c_int()
+        self._check(lib().ds_mlp_get_prior(self._h, None, 0, C.byref(n)), "ds_mlp_get_prior")
+        p = np.zeros(n.value, np.float32)
+        if n.value:
+            self._check(lib().ds_mlp_get_prior(self._h, ptr(p), n.value, C.byref(n)), "ds_mlp_get_prior")
+        return p
 
     def infer(self, F):
         F = np.ascontiguousarray(F, np.float32)

@@ -94,6 +94,7 @@ def test_features_of_a_hand_worked_lp():
         [1, 0, 1.5, r5, 2, 1, 2 / 3, 0.75, 5 / 3, 0.625, 0.5, 0.5, 0, 0, 0, 0],
         [1, 0, 1.5, r5, 2, 1, 4 / 3, 0.375, 2 / 3, 0.3125, 0, 0, 0, 0, 0, 0],
     ])
+    expect = np.hstack([expect, np.zeros((5, 1))])  # column 16, the family prior, is filled only at prediction
     assert np.allclose(F, expect, atol=1e-12), np.argwhere(~np.isclose(F, expect, atol=1e-12))
 
 
@@ -154,3 +155,13 @@ def test_slack_substitution_repairs_dependent_columns():
         if nrep:
             with pytest.raises(ValueError):
                 W.gj_inverse(A, start, n, 0.0)
+
+
+def test_family_prior_by_hand():
+    """Three training bases over four variables: variable 0 basic in all, 1 in two,
+    2 in one, 3 in none."""
+    Y = np.array([[1, 1, 0, 0], [1, 0, 1, 0], [1, 1, 0, 0]], np.float64)
+    np.testing.assert_allclose(W.family_prior(Y, loo=False), [1.0, 2 / 3, 1 / 3, 0.0])
+    # leave one out: each training row sees the other two bases only
+    np.testing.assert_allclose(W.family_prior(Y, loo=True),
+                               [[1.0, 0.5, 0.5, 0.0], [1.0, 1.0, 0.0, 0.0], [1.0, 0.5, 0.5, 0.0]])
