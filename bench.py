@@ -426,7 +426,7 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
     warm_prm = default_params(refactor_freq=refactor_freq, time_limit_ms=limit_ms, warm_primal=warm_primal)
     with BatchSolver(train) as T, Mlp(seed=1) as net:
         T.set_bases(bases)
-        losses, fit_ms = net.fit(T, epochs=epochs, minibatch=8192, lr=1e-3, seed=3)
+        losses, fit_ms = net.fit(T, epochs=epochs, minibatch=2048, lr=1e-3, seed=3)
         log("learned warm start: losses %s" % [round(float(v), 4) for v in losses])
         tr_ms, tr_flops = net.train_stats()
         with BatchSolver(held) as E:
@@ -589,7 +589,7 @@ def main():
     ap.add_argument("--train", type=int, default=16, help="GPU-solved training instances of the family")
     ap.add_argument("--threads", type=int, default=4, help="family LPs solved at a time (contexts/streams)")
     ap.add_argument("--family-refactor", type=int, default=150)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=48, help="training epochs of the family network (minibatch 2048)")
     ap.add_argument("--family-cpu-iters", type=int, default=300)
     ap.add_argument("--warm-primal", type=int, default=1,
                     help="1: primal simplex iterations from the bound-shifted predicted basis before the dual simplex")

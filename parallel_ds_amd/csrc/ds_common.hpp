@@ -70,8 +70,6 @@ struct DevScalars {
   int ctr[16];       // last-workgroup counters, reset by the finaliser
 };
 
-constexpr int MAX_BLOCKED = 64;
-
 // Counter slots in DevScalars::ctr.
 enum { CTR_PRICE = 0, CTR_CHUZC1, CTR_CHUZC2, CTR_DUPD, CTR_UPD, CTR_DCHK, CTR_DOT, CTR_INF, CTR_SPMV, CTR_FLIP, CTR_BXB };
 

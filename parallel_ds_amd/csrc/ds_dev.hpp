@@ -60,7 +60,7 @@ struct DevIter {
   double* devex;           // N: Devex reference weights by variable
   int eta_cap;
   unsigned char* blocked;  // N
-  int* blocked_list;       // MAX_BLOCKED
+  int* blocked_list;       // m: rows blocked by phase 1 since the last non-empty ratio test
   int* list_idx;           // BTRAN right-hand-side list (kernel index)
   double* list_val;
   int* ylist_row;          // BTRAN slack-row list

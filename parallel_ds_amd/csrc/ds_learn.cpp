@@ -31,13 +31,6 @@ static bool family_shape(const BatchSolver& B, int& N) {
   return true;
 }
 
-// Writes column 16 of the device feature matrix F (R rows) from a host column.
-static void put_prior_column(hipStream_t st, float* F, const std::vector<float>& col) {
-  PDS_HIP(hipMemcpy2DAsync(F + (WS_NF - 1), sizeof(float) * WS_NF, col.data(), sizeof(float), sizeof(float),
-                           col.size(), hipMemcpyHostToDevice, st));
-  PDS_HIP(hipStreamSynchronize(st));
-}
-
 double fit_from_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, int epochs, int minibatch, float lr,
                       unsigned long long seed, double* epoch_loss) {
   if (net.n_features() != WS_NF) throw std::invalid_argument("network must take the batch features (17)");
@@ -53,18 +46,17 @@ double fit_from_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, int epochs, int
   int N = 0;
   std::vector<float> prior;
   if (B.count() > 1 && family_shape(B, N)) {
+    // Label counts are integers below 2^24: exact in FP32.
     const int T = B.count();
-    std::vector<float> y(R), col(R);
-    PDS_HIP(hipMemcpyAsync(y.data(), wb.y.p, sizeof(float) * R, hipMemcpyDeviceToHost, B.stream()));
+    wb.prior.alloc(N);
+    launch_prior_count(B.stream(), wb.y.p, T, N, wb.prior.p);
+    launch_put_prior(B.stream(), wb.F.p, wb.prior.p, wb.y.p, R, N, (float)(T - 1));
+    PDS_HIP(hipGetLastError());
+    std::vector<float> cnt(N);
+    PDS_HIP(hipMemcpyAsync(cnt.data(), wb.prior.p, sizeof(float) * N, hipMemcpyDeviceToHost, B.stream()));
     PDS_HIP(hipStreamSynchronize(B.stream()));
-    std::vector<double> cnt(N, 0.0);
-    for (int k = 0; k < T; ++k)
-      for (int j = 0; j < N; ++j) cnt[j] += y[(size_t)k * N + j];
-    for (int k = 0; k < T; ++k)
-      for (int j = 0; j < N; ++j) col[(size_t)k * N + j] = (float)((cnt[j] - y[(size_t)k * N + j]) / (T - 1));
-    put_prior_column(B.stream(), wb.F.p, col);
     prior.resize(N);
-    for (int j = 0; j < N; ++j) prior[j] = (float)(cnt[j] / T);
+    for (int j = 0; j < N; ++j) prior[j] = cnt[j] / (float)T;
   }
   net.set_prior(prior);
   launch_colstats(B.stream(), wb.F.p, R, wb.mu.p, wb.sd.p);
@@ -103,9 +95,10 @@ double warm_start_batch(Mlp& net, BatchSolver& B, WarmBuffers& wb, double* infer
   int N = 0;
   const std::vector<float>& pr = net.prior();
   if (!pr.empty() && family_shape(B, N) && N == (int)pr.size()) {
-    std::vector<float> col(R);
-    for (long long r = 0; r < R; ++r) col[r] = pr[r % N];
-    put_prior_column(B.stream(), wb.F.p, col);
+    wb.prior.alloc(N);
+    PDS_HIP(hipMemcpyAsync(wb.prior.p, pr.data(), sizeof(float) * N, hipMemcpyHostToDevice, B.stream()));
+    launch_put_prior(B.stream(), wb.F.p, wb.prior.p, nullptr, R, N, 1.f);
+    PDS_HIP(hipGetLastError());
   }
   PDS_HIP(hipStreamSynchronize(B.stream()));
   net.infer(wb.F.p, nullptr, (int)R, wb.z.p);  // synchronises its stream

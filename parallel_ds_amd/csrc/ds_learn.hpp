@@ -7,7 +7,7 @@
 namespace pds {
 
 struct WarmBuffers {
-  DBuf<float> F, y, z, mu, sd;
+  DBuf<float> F, y, z, mu, sd, prior;
   DBuf<int> order;
 };
 

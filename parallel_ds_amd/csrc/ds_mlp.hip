@@ -16,6 +16,8 @@
 // for e < 4.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ds_mlp.hpp"
 #include "ds_reduce.hpp"
 
@@ -38,10 +40,12 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
 
 // 256 threads, a 64 x 64 tile of C per workgroup, 32 x 32 per wavefront
 // (2 x 2 MFMA tiles).  M, N multiples of 32 (16-row tiles beyond M or N are
-// skipped per wavefront), K a multiple of 32.
+// skipped per wavefront), K a multiple of 32.  Workgroup z takes K columns
+// [z kc, (z + 1) kc): with the FP32 epilogue and gridDim.z > 1 it writes the
+// partial product of its slice to C32 + z M N (split K, summed by k_splitk_sum).
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, const bf16_t* __restrict__ B, int M,
-                                                 int N, int K, const float* __restrict__ bias,
+                                                 int N, int K, int kc, const float* __restrict__ bias,
                                                  const bf16_t* __restrict__ mask, bf16_t* __restrict__ C16,
                                                  bf16_t* __restrict__ CT16, float* __restrict__ C32) {
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
@@ -57,7 +61,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, c
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+  if (EPI == MLP_EPI_F32) C32 += (size_t)blockIdx.z * M * N;
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     bf8 fa[2], fb[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -100,6 +106,15 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, c
         *reinterpret_cast<uint2*>(CT16 + (size_t)col * M + row0) = pk;
       }
     }
+}
+
+// C[i] = sum over z < splits of P[z MN + i], in ascending z.
+__global__ void k_splitk_sum(const float* __restrict__ P, int splits, int MN, float* __restrict__ C) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += P[(size_t)z * MN + i];
+  C[i] = s;
 }
 
 // Output layer and loss.  One wavefront per sample: z = w4 . h3 + b4; with
@@ -288,21 +303,39 @@ __global__ __launch_bounds__(256) void k_infer(const bf16_t* __restrict__ X, int
 }  // namespace
 
 void mlp_gemm_nt(hipStream_t st, int epi, const bf16_t* A, const bf16_t* B, int M, int N, int K, const float* bias,
-                 const bf16_t* mask, bf16_t* C16, bf16_t* CT16, float* C32, hipEvent_t e0, hipEvent_t e1) {
+                 const bf16_t* mask, bf16_t* C16, bf16_t* CT16, float* C32, hipEvent_t e0, hipEvent_t e1, float* ws,
+                 size_t ws_floats) {
   dim3 grid((N + 63) / 64, (M + 63) / 64);
   switch (epi) {
     case MLP_EPI_BIAS_RELU:
-      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_BIAS_RELU>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, bias, mask,
-                            C16, CT16, C32);
+      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_BIAS_RELU>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, K, bias,
+                            mask, C16, CT16, C32);
       break;
     case MLP_EPI_MASK:
-      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_MASK>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, bias, mask, C16,
-                            CT16, C32);
+      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_MASK>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, K, bias, mask,
+                            C16, CT16, C32);
       break;
-    default:
-      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_F32>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, bias, mask, C16,
-                            CT16, C32);
+    default: {
+      // Weight gradients (long K = samples, few output tiles): split K so that
+      // the grid has about MLP_SPLITK_WGS workgroups, each slice a multiple of 32.
+      const int tiles = (int)(grid.x * grid.y);
+      int splits = (MLP_SPLITK_WGS + tiles - 1) / tiles;
+      splits = std::min(splits, K / 32);
+      while (splits > 1 && (size_t)splits * M * N > ws_floats) --splits;
+      if (!ws || splits <= 1) {
+        hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_F32>, grid, dim3(256), 0, st, e0, e1, 0, A, B, M, N, K, K, bias, mask,
+                              C16, CT16, C32);
+        break;
+      }
+      const int kc = ((K + splits - 1) / splits + 31) / 32 * 32;
+      splits = (K + kc - 1) / kc;
+      grid.z = splits;
+      hipExtLaunchKernelGGL(k_gemm_nt<MLP_EPI_F32>, grid, dim3(256), 0, st, e0, nullptr, 0, A, B, M, N, K, kc, bias,
+                            mask, C16, CT16, ws);
+      hipExtLaunchKernelGGL(k_splitk_sum, dim3((M * N + 255) / 256), dim3(256), 0, st, nullptr, e1, 0, ws, splits,
+                            M * N, C32);
       break;
+    }
   }
 }
 

@@ -16,9 +16,13 @@ constexpr int MLP_WIDTH = 256;   // hidden width
 constexpr int MLP_KPAD = 32;     // input features padded to one MFMA K-step
 constexpr int MLP_EPI_BIAS_RELU = 0, MLP_EPI_MASK = 1, MLP_EPI_F32 = 2;
 
+constexpr int MLP_SPLITK_WGS = 256;            // target grid of a split-K weight-gradient GEMM
+constexpr size_t MLP_SPLITK_FLOATS = 1u << 21;  // its partial-product workspace (8 MiB)
+// C = A B^T.  With the FP32 epilogue and a workspace of ws_floats floats, K is
+// split over the grid's z dimension and the partials summed in a second launch.
 void mlp_gemm_nt(hipStream_t st, int epi, const bf16_t* A, const bf16_t* B, int M, int N, int K, const float* bias,
                  const bf16_t* mask, bf16_t* C16, bf16_t* CT16, float* C32, hipEvent_t e0 = nullptr,
-                 hipEvent_t e1 = nullptr);
+                 hipEvent_t e1 = nullptr, float* ws = nullptr, size_t ws_floats = 0);
 void mlp_head(hipStream_t st, const bf16_t* H3, const float* w4, const float* b4, int S, int Sv, const int* rows,
               const float* y, const float* wt, float inv_s, float* z, float* dz, float* loss_part, bf16_t* G3,
               bf16_t* G3T);
@@ -81,6 +85,7 @@ class Mlp {
   hipStream_t st_ = nullptr;
   long long step_ = 0;
   DBuf<float> p_, g_, m1_, m2_, mu_, sd_, z_, dz_, loss_, scal_;
+  DBuf<float> ws_;  // split-K partial products of the weight gradients (MLP_SPLITK_FLOATS)
   DBuf<bf16_t> W1b_, W1Tb_, W2b_, W2Tb_, W3b_, W3Tb_;
   DBuf<bf16_t> X_, XT_, H1_, H1T_, H2_, H2T_, H3_, H3T_, G3_, G3T_, G2_, G2T_, G1_, G1T_;
   int cap_ = 0;

@@ -28,6 +28,7 @@ Mlp::Mlp(int device, int n_features, unsigned long long seed) : device_(device),
   mu_.alloc(MLP_KPAD);
   sd_.alloc(MLP_KPAD);
   scal_.alloc(4);
+  ws_.alloc(MLP_SPLITK_FLOATS);
   const int W = MLP_WIDTH, K = MLP_KPAD;
   W1b_.alloc((size_t)W * K);
   W1Tb_.alloc((size_t)W * K);
@@ -141,7 +142,7 @@ static void enqueue_grad(hipStream_t st, int nf, const float* F, const int* rows
                          bf16_t* XT, const bf16_t* W2T, const bf16_t* W3T, const bf16_t* W1, const bf16_t* W2,
                          const bf16_t* W3, bf16_t* H1, bf16_t* H1T, bf16_t* H2, bf16_t* H2T, bf16_t* H3, bf16_t* H3T,
                          bf16_t* G3, bf16_t* G3T, bf16_t* G2, bf16_t* G2T, bf16_t* G1, bf16_t* G1T, float* z,
-                         float* dz, float* loss_part, float* loss_out) {
+                         float* dz, float* loss_part, float* loss_out, float* ws) {
   using L = MlpLayout;
   const int W = MLP_WIDTH, K = MLP_KPAD;
   const int Sp = round_up(S, 64);
@@ -152,13 +153,16 @@ static void enqueue_grad(hipStream_t st, int nf, const float* F, const int* rows
   mlp_head(st, H3, p + L::ow4, p + L::ob4, Sp, S, rows, y, wt, 1.f / (float)S, z, dz, loss_part, G3, G3T);
   mlp_rowsum(st, H3T, W, Sp, dz, g + L::ow4);
   mlp_sum(st, dz, Sp, g + L::ob4);
-  mlp_gemm_nt(st, MLP_EPI_F32, G3T, H2T, W, W, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW3);
+  mlp_gemm_nt(st, MLP_EPI_F32, G3T, H2T, W, W, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW3, nullptr, nullptr, ws,
+              MLP_SPLITK_FLOATS);
   mlp_rowsum(st, G3T, W, Sp, nullptr, g + L::ob3);
   mlp_gemm_nt(st, MLP_EPI_MASK, G3, W3T, Sp, W, W, nullptr, H2, G2, G2T, nullptr);
-  mlp_gemm_nt(st, MLP_EPI_F32, G2T, H1T, W, W, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW2);
+  mlp_gemm_nt(st, MLP_EPI_F32, G2T, H1T, W, W, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW2, nullptr, nullptr, ws,
+              MLP_SPLITK_FLOATS);
   mlp_rowsum(st, G2T, W, Sp, nullptr, g + L::ob2);
   mlp_gemm_nt(st, MLP_EPI_MASK, G2, W2T, Sp, W, W, nullptr, H1, G1, G1T, nullptr);
-  mlp_gemm_nt(st, MLP_EPI_F32, G1T, XT, W, K, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW1);
+  mlp_gemm_nt(st, MLP_EPI_F32, G1T, XT, W, K, Sp, nullptr, nullptr, nullptr, nullptr, g + L::oW1, nullptr, nullptr, ws,
+              MLP_SPLITK_FLOATS);
   mlp_rowsum(st, G1T, W, Sp, nullptr, g + L::ob1);
   mlp_sum(st, loss_part, Sp / 4, loss_out);
 }
@@ -168,7 +172,7 @@ double Mlp::grad(const float* F, const int* rows, int S, const float* y, const f
   reserve(S);
   enqueue_grad(st_, nf_, F, rows, S, y, wt, p_.p, g_.p, mu_.p, sd_.p, X_.p, XT_.p, W2Tb_.p, W3Tb_.p, W1b_.p, W2b_.p,
                W3b_.p, H1_.p, H1T_.p, H2_.p, H2T_.p, H3_.p, H3T_.p, G3_.p, G3T_.p, G2_.p, G2T_.p, G1_.p, G1T_.p, z_.p,
-               dz_.p, loss_.p, scal_.p);
+               dz_.p, loss_.p, scal_.p, ws_.p);
   PDS_HIP(hipGetLastError());
   float loss = 0.f;
   PDS_HIP(hipMemcpyAsync(&loss, scal_.p, sizeof(float), hipMemcpyDeviceToHost, st_));
@@ -194,7 +198,7 @@ double Mlp::train_epoch(const float* F, const int* order, int S_total, const flo
     const int off = s * batch, S = std::min(batch, S_total - off);
     enqueue_grad(st_, nf_, F, order + off, S, y, nullptr, p_.p, g_.p, mu_.p, sd_.p, X_.p, XT_.p, W2Tb_.p, W3Tb_.p,
                  W1b_.p, W2b_.p, W3b_.p, H1_.p, H1T_.p, H2_.p, H2T_.p, H3_.p, H3T_.p, G3_.p, G3T_.p, G2_.p, G2T_.p,
-                 G1_.p, G1T_.p, z_.p, dz_.p, loss_.p, losses.p + s);
+                 G1_.p, G1T_.p, z_.p, dz_.p, loss_.p, losses.p + s, ws_.p);
     adam_step(lr);
     gemm_flops_ += 2LL * round_up(S, 64) * MLP_WIDTH * (2LL * MLP_KPAD + 6LL * MLP_WIDTH);
   }

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "ds_rng.hpp"
@@ -348,7 +349,7 @@ void Solver::load(const HostModel& hm) {
   nonbasis_.alloc(n);
   posN_.alloc(N);
   blocked_.alloc(N);
-  blocked_list_.alloc(MAX_BLOCKED);
+  blocked_list_.alloc(std::max(m, 1));
   list_idx_.alloc(1 << 17);
   list_val_.alloc(1 << 17);
   ylist_row_.alloc(ETA_MAX + 1);
@@ -925,6 +926,7 @@ int Solver::phase1() {
     hS_->status = ST_RUNNING;
     hS_->refac_mid = 0;
     push();
+    if (P_.verbose) std::fprintf(stderr, "[ds] phase 1: event %d at iteration %d\n", st, stop);
     prof_host_begin();
     switch (st) {
       case ST_PH1_SOLVED:
@@ -953,12 +955,7 @@ int Solver::phase1() {
         phase1Init();
         initBetaWeights(false);
         pull();
-        std::vector<int> lst(hS_->nblocked);
-        if (!lst.empty()) {
-          PDS_HIP(hipMemcpy(lst.data(), blocked_list_.p, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
-          unsigned char z = 0;
-          for (int j : lst) PDS_HIP(hipMemcpy(blocked_.p + j, &z, 1, hipMemcpyHostToDevice));
-        }
+        if (hS_->nblocked > 0) PDS_HIP(hipMemsetAsync(blocked_.p, 0, H_.N, st_));
         hS_->nblocked = 0;
         hS_->any_blocked = 0;
         push();
@@ -970,7 +967,7 @@ int Solver::phase1() {
         int p = hS_->p;
         unsigned char one = 1;
         PDS_HIP(hipMemcpy(blocked_.p + p, &one, 1, hipMemcpyHostToDevice));
-        if (hS_->nblocked < MAX_BLOCKED) {
+        if (hS_->nblocked < H_.m) {  // a blocked variable is not priced again until the set clears
           PDS_HIP(hipMemcpy(blocked_list_.p + hS_->nblocked, &p, sizeof(int), hipMemcpyHostToDevice));
           hS_->nblocked++;
         }
@@ -1044,6 +1041,7 @@ bool Solver::phase2() {
     hS_->status = ST_RUNNING;
     hS_->refac_mid = 0;
     push();
+    if (P_.verbose) std::fprintf(stderr, "[ds] phase 2: event %d at iteration %d\n", st, stop);
     switch (st) {
       case ST_OPTIMAL:
         solved_ = true;

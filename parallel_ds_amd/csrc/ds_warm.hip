@@ -233,6 +233,25 @@ __global__ __launch_bounds__(WBS) void k_colstats(const float* __restrict__ F, l
   }
 }
 
+// Family prior (last feature) of a batch of T LPs with N variables each:
+// cnt[j] = sum over the LPs of y[k N + j].
+__global__ void k_prior_count(const float* __restrict__ y, int T, int N, float* __restrict__ cnt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.f;
+  for (int k = 0; k < T; ++k) s += y[(size_t)k * N + j];
+  cnt[j] = s;
+}
+
+// F[r][WS_NF - 1] = (base[r mod N] - y[r]) / denom (y may be null: 0).
+__global__ void k_put_prior(float* __restrict__ F, const float* __restrict__ base, const float* __restrict__ y,
+                            long long R, int N, float denom) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const float v = base[r % N] - (y ? y[r] : 0.f);
+  F[r * WS_NF + (WS_NF - 1)] = v / denom;
+}
+
 }  // namespace
 
 size_t features_lds_bytes(int mmax, int nmax) { return sizeof(float) * (3 * (size_t)nmax + 3 * (size_t)mmax); }
@@ -251,6 +270,12 @@ void launch_labels(hipStream_t s, const BatchDev& D, int count, const int* basis
 }
 void launch_select(hipStream_t s, const BatchDev& D, int count, int Nmax, const float* z, int* warm) {
   k_select<<<count, WBS, select_lds_bytes(Nmax), s>>>(D, z, warm);
+}
+void launch_prior_count(hipStream_t s, const float* y, int T, int N, float* cnt) {
+  k_prior_count<<<(N + WBS - 1) / WBS, WBS, 0, s>>>(y, T, N, cnt);
+}
+void launch_put_prior(hipStream_t s, float* F, const float* base, const float* y, long long R, int N, float denom) {
+  k_put_prior<<<(unsigned)((R + WBS - 1) / WBS), WBS, 0, s>>>(F, base, y, R, N, denom);
 }
 void launch_colstats(hipStream_t s, const float* F, long long R, float* mu, float* sd) {
   k_colstats<<<WS_NF, WBS, 0, s>>>(F, R, mu, sd);
