@@ -396,7 +396,7 @@ def _solve_many(items, prm_of, threads):
 
 
 def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_iters, cpu_from=1500, m=2000,
-                    n=5000, k=5, seed=4, sigma=0.1, slack_limit_s=30.0, warm_primal=1):
+                    n=5000, k=5, seed=4, sigma=0.1, slack_limit_s=30.0, warm_primal=1, profile_part=None):
     """Learned warm start against slack start at the family's stated size: the single-LP recipe at
     m=2,000, n=5,000 with 5 nonzeros per column (base seed 4), b and c times lognormal(0, 0.1).
     The network is trained on the optimal bases the GPU solver finds for n_train instances (seeds
@@ -404,7 +404,9 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
     (the next seeds) are solved from the slack basis and from the learned warm start, `threads` LPs
     at a time (one context and HIP stream each). A warm solve that has not ended optimal within
     twice the mean training solve time is re-solved from the slack basis (the fallback the slack
-    start provides); its time counts in full."""
+    start provides); its time counts in full.  profile_part "train" stops after training, "warm"
+    solves every held-out instance from the learned start only (kernel profiles of the two parts:
+    the warm solves' kernels are the difference of the two traces)."""
     import numpy as np
     from parallel_ds_amd import default_params
     from parallel_ds_amd import lp as lpmod
@@ -435,6 +437,11 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
             pred_s = time.perf_counter() - t0
             warm = E.get_warm()
     overlap = []
+    part = {"workload": "LP family m=%d n=%d: %s only (profiling run, no headline)" % (m, n, profile_part),
+            "learned_warm": {"solves_per_s": None, "wall_s": 0.0}, "slack": {"solves_per_s": None},
+            "speedup_solves_per_s": None}
+    if profile_part == "train":
+        return part
 
     def warm_phase(lps, starts):
         t0 = time.perf_counter()
@@ -446,6 +453,12 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
                 rw[i] = r
         return time.perf_counter() - t0, rw, len(redo)
 
+    if profile_part == "warm":
+        log("learned warm start: %d held-out instances from the learned warm start only" % len(held))
+        wall_w, rw, fallbacks = warm_phase(held, warm)
+        part["learned_warm"].update({"wall_s": round(wall_w, 3), "solves": len(held), "fallbacks_to_slack": fallbacks,
+                                     "mean_iterations": round(float(np.mean([r[3] for r in rw])), 1)})
+        return part
     if warmup > 0:
         log("learned warm start: %d untimed held-out instances" % warmup)
         _solve_many([(lp, None) for lp in held[:warmup]], lambda i: slack_prm, threads)
@@ -484,7 +497,7 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
         "predicted_basis_overlap_with_optimal": round(float(np.mean(overlap)), 4),
         "all_optimal": bool(ok), "optimal_both": len(both), "slack_not_optimal": sum(a[1] != 0 for a in rs),
         "max_obj_rel_err_warm_vs_slack": float(rel),
-        "training": {"instances": n_train, "optimal": len(keep), "solve_wall_s": round(t_tr, 2),
+        "training": {"instances": n_train, "optimal": len(keep), "solve_wall_s": round(t_tr, 2), "minibatch": 2048,
                      "mean_iterations": round(float(np.mean([r[3] for r in tr])), 1),
                      "epochs": epochs, "epoch_loss": [round(float(v), 4) for v in losses], "fit_ms": round(fit_ms, 1),
                      "train_gemm_tflops": round(tr_flops / (tr_ms * 1e-3) / 1e12, 2) if tr_ms > 0 else None},
@@ -591,6 +604,9 @@ def main():
     ap.add_argument("--family-refactor", type=int, default=150)
     ap.add_argument("--epochs", type=int, default=48, help="training epochs of the family network (minibatch 2048)")
     ap.add_argument("--family-cpu-iters", type=int, default=300)
+    ap.add_argument("--family-profile", choices=["train", "warm"], default=None,
+                    help="profiling runs of the family leg: stop after training, or solve the held-out "
+                         "instances from the learned start only (no headline value)")
     ap.add_argument("--warm-primal", type=int, default=1,
                     help="1: primal simplex iterations from the bound-shifted predicted basis before the dual simplex")
     ap.add_argument("--m", type=int, default=20000)
@@ -631,7 +647,8 @@ def main():
                "speedup_solves_per_s": None, "workload": "not run (--no-family)"}
     else:
         fam = learned_warm_c4(args.steps, args.warmup, args.train, args.threads, args.family_refactor, args.epochs,
-                              0 if args.no_cpu_baseline else args.family_cpu_iters, warm_primal=args.warm_primal)
+                              0 if args.no_cpu_baseline else args.family_cpu_iters, warm_primal=args.warm_primal,
+                              profile_part=args.family_profile)
     single = None if args.no_single else single_lp_window(args)
     out = {
         "metric": METRIC,
@@ -640,7 +657,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * fam["learned_warm"]["wall_s"] / args.steps, 3),
+        "ms_per_step": round(1e3 * fam["learned_warm"]["wall_s"] / max(args.steps, 1), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
