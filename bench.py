@@ -494,6 +494,8 @@ def learned_warm_c4(steps, warmup, n_train, threads, refactor_freq, epochs, cpu_
                          "repaired_columns_mean": round(float(np.mean([r[4] for r in rw])), 2),
                          "predict_s": round(pred_s, 4)},
         "speedup_solves_per_s": round(wall_s / wall_w, 4),
+        # like-for-like per solve: summed solve times over the instances both starts solved to optimality
+        "speedup_solve_ms_both_optimal": round(sum(a[6] for a, b in both) / max(1e-9, sum(b[6] for a, b in both)), 4),
         "predicted_basis_overlap_with_optimal": round(float(np.mean(overlap)), 4),
         "all_optimal": bool(ok), "optimal_both": len(both), "slack_not_optimal": sum(a[1] != 0 for a in rs),
         "max_obj_rel_err_warm_vs_slack": float(rel),

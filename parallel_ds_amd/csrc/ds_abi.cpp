@@ -344,7 +344,7 @@ int ds_set_basis(ds_ctx* ctx, const int* basis) {
 }
 
 int ds_refactor(ds_ctx* ctx) {
-  return guard(ctx, [&](pds::Solver& s) { s.refactor(); });
+  return guard(ctx, [&](pds::Solver& s) { s.refactor(true); });
 }
 
 int ds_btran(ds_ctx* ctx, int p_idx, double* rho) {

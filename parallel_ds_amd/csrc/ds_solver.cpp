@@ -413,7 +413,15 @@ static void ensure_kernel_capacity(DBuf<double>& b, int k) {
   }
 }
 
-void Solver::refactor() {
+void Solver::refactor(bool force) {
+  // No basis change since the last factorization (no eta): a new factor of the
+  // same basis would be bit-for-bit the one already held (the kernels are
+  // deterministic), so the restart paths that refactor again skip the work.
+  if (!force && factor_valid_ && t_ == 0 && !(P_.force_event == DS_FORCE_SINGULAR && !force_fired_)) {
+    refactors_skipped_++;
+    return;
+  }
+  factor_valid_ = false;
   const double t0 = now_ms();
   // A numerically singular kernel either ends the call (repair off) or has the
   // slacks of the rows chosen for its dependent columns substituted for those
@@ -427,6 +435,7 @@ void Solver::refactor() {
   }
   refactors_++;
   refactor_ms_ += now_ms() - t0;
+  factor_valid_ = true;
 }
 
 // Basis position pos_P[g] of each dependent kernel column g takes the slack of
@@ -677,6 +686,7 @@ void Solver::initDualSimplex() {
   upload(basis_, basis, st_);
   upload(nonbasis_, nonbasis, st_);
   upload(posN_, posN, st_);
+  factor_valid_ = false;
   launch_fill(st_, x_.p, 0.0, N);
   launch_fill(st_, d_.p, 0.0, N);
   launch_fill(st_, beta_.p, 1.0, m);
@@ -1459,6 +1469,7 @@ void Solver::install_basis(const int* basis) {
   upload(nonbasis_, nb, st_);
   upload(posN_, posN, st_);
   sync();
+  factor_valid_ = false;
 }
 
 void Solver::btran(int p_idx, double* rho) {
@@ -1535,7 +1546,7 @@ void Solver::time_pivot_row_ext(const double* rho, int reps, long long flush_byt
 // refactorization, and the device time and executed flops (2 (k - nb) k nb per
 // panel of nb columns) of its FP64 MFMA trailing updates from event pairs.
 void Solver::time_refactor(int reps, double& refactor_ms, double& gemm_ms, double& gemm_flops, int& k) {
-  refactor();
+  refactor(true);
   k = k_;
   const int panels = (k_ + GJ_NB - 1) / GJ_NB;
   for (int i = (int)gemm_ev_.size(); i < 2 * panels; ++i) {
@@ -1547,7 +1558,7 @@ void Solver::time_refactor(int reps, double& refactor_ms, double& gemm_ms, doubl
   for (int r = 0; r < reps; ++r) {
     sync();
     const double t0 = now_ms();
-    refactor();
+    refactor(true);
     sync();
     tr += now_ms() - t0;
     for (int p = 0; p < panels; ++p) {

@@ -84,7 +84,7 @@ class Solver {
   void set_basis(const int* basis);
   void init_basis(const int* basis);
   void install_basis(const int* basis);
-  void refactor();
+  void refactor(bool force = false);  // force: factor again even when the basis is unchanged
   void btran(int p_idx, double* rho);
   void ftran(const double* b, double* x);
   void ftran_column(int j, double* x);
@@ -121,6 +121,8 @@ class Solver {
   DBuf<int> blist_, flist_;
   DBuf<double> bcc_, dxb_;
   bool has_boxed_ = false;
+  bool factor_valid_ = false;  // the factor is that of the current basis (no eta since)
+  int refactors_skipped_ = 0;
   // Primal simplex iterations of a warm start (params.warm_primal).
   bool primal_ok_ = true;        // no Free or Boxed variables
   DBuf<double> devex_;
