@@ -396,17 +396,18 @@ constexpr int SPMV_COLS_MAX = 512;
 constexpr int SPMV_SEG = 2048;
 constexpr int SPMV_ILP = 4;
 constexpr int SPMV_GRID = 1024;  // tile width: the fewest columns that keep the grid within this
-__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __restrict__ posN,
-                                                       const double* __restrict__ v, double alpha, double beta,
-                                                       const double* __restrict__ cvec, double* __restrict__ out,
-                                                       int by_col, const DevScalars* S, int check, int cols) {
+// bid: the tile (workgroup) index within the SpMV's share of the launch.
+__device__ __forceinline__ void spmv_body(const DevModel& M, const int* __restrict__ posN, const double* __restrict__ v,
+                                          double alpha, double beta, const double* __restrict__ cvec,
+                                          double* __restrict__ out, int by_col, const DevScalars* S, int check,
+                                          int cols, int bid) {
   __shared__ int cp[SPMV_COLS_MAX + 1];
   __shared__ int pn[SPMV_COLS_MAX];
   __shared__ double cb[SPMV_COLS_MAX];
   __shared__ double prod[SPMV_SEG];
   __shared__ int halt;
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * cols;
+  const int j0 = bid * cols;
   const int ncol = min(cols, M.N - j0);
   // Everything that does not depend on the entries is fetched in one round.
   for (int c = tid; c <= ncol; c += SPMV_THREADS) cp[c] = M.cptr[j0 + c];
@@ -448,6 +449,13 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __
     }
     out[by_col ? j0 + c : pn[c]] = acc + cb[c] * beta;
   }
+}
+
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv(DevModel M, const int* __restrict__ posN,
+                                                       const double* __restrict__ v, double alpha, double beta,
+                                                       const double* __restrict__ cvec, double* __restrict__ out,
+                                                       int by_col, const DevScalars* S, int check, int cols) {
+  spmv_body(M, posN, v, alpha, beta, cvec, out, by_col, S, check, cols, blockIdx.x);
 }
 
 // Tile width: the smallest power of two in [64, SPMV_COLS_MAX] whose grid has
@@ -517,14 +525,15 @@ __device__ inline void chuzc_finish(const DevModel& M, const DevIter& I, int pha
 // workgroup whose list overflowed and may hold global candidates makes the
 // last workgroup rescan all positions.
 constexpr int TB_LIST = 8;
-__global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phase, int it) {
+// bid, nb: this workgroup and the workgroup count of the CHUZC share of the launch.
+__device__ __forceinline__ void chuzc_tb_body(const DevModel& M, const DevIter& I, int phase, int it, int bid, int nb) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
   const int neg = S->neg;
   const int nN = M.N - M.m;
   double mn = BIG;
   int nF = 0;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+  for (int i = bid * BS + threadIdx.x; i < nN; i += nb * BS) {
     double a = I.alpha_r[i];
     if (neg) a = -a;
     int j = I.nonbasis[i];
@@ -539,7 +548,7 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
   __shared__ double lt[TB_LIST], la[TB_LIST];
   if (threadIdx.x == 0) lc = 0;
   __syncthreads();
-  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+  for (int i = bid * BS + threadIdx.x; i < nN; i += nb * BS) {
     double a = I.alpha_r[i];
     if (neg) a = -a;
     int j = I.nonbasis[i];
@@ -555,7 +564,7 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
     }
   }
   __syncthreads();
-  const int b = blockIdx.x;
+  const int b = bid;
   if (threadIdx.x == 0) {
     I.part_d[b] = mn;
     I.part_i[2 * b] = nF;
@@ -567,10 +576,10 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
       I.cand_th[b * TB_LIST + e] = lt[e];
       I.cand_a[b * TB_LIST + e] = la[e];
     }
-  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  if (!last_block(&S->ctr[CTR_CHUZC1], nb)) return;
   double g = BIG;
   int tot = 0;
-  for (int w = threadIdx.x; w < (int)gridDim.x; w += BS) {
+  for (int w = threadIdx.x; w < nb; w += BS) {
     g = fmin(g, I.part_d[w]);
     tot += I.part_i[2 * w];
   }
@@ -579,7 +588,7 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
   int rescan = 0;
   double best = -1.0;
   int bi = INT_MAX;
-  for (int w = threadIdx.x; w < (int)gridDim.x; w += BS) {
+  for (int w = threadIdx.x; w < nb; w += BS) {
     int n = I.part_i[2 * w + 1];
     if (n > TB_LIST && I.part_d[w] < g + EPS_Z) rescan = 1;
     n = n < TB_LIST ? n : TB_LIST;
@@ -608,6 +617,10 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
     return;
   }
   chuzc_finish(M, I, phase, it, bi);
+}
+
+__global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phase, int it) {
+  chuzc_tb_body(M, I, phase, it, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase, int harris, int it) {
@@ -687,8 +700,8 @@ __global__ __launch_bounds__(BS) void k_chuzc2(DevModel M, DevIter I, int phase,
 // select rows of K^{-T}; slack positions subtract the coupling block.
 // ---------------------------------------------------------------------------
 constexpr int FT_MAXCOL = 512;
-__global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFactor F, int col, double* out,
-                                                  int check) {
+__device__ __forceinline__ void ftran_col_body(const DevModel& M, const DevIter& I, const DevFactor& F, int col,
+                                               double* out, int check, int bid) {
   DevScalars* S = I.S;
   if (skip_launch(S, check)) return;
   const int j = col >= 0 ? col : S->q;
@@ -696,7 +709,7 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   __shared__ int eri[FT_MAXCOL];
   __shared__ double ev[FT_MAXCOL];
   // The row's own factor data does not depend on the column: fetch it first.
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = bid * BS + threadIdx.x;
   const bool live = i < M.m;
   const int c = live ? F.Pidx_of_pos[i] : 0;
   const int r = (live && c < 0) ? F.slackrow_of_pos[i] : 0;
@@ -752,13 +765,18 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   out[i] = x;
 }
 
+__global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFactor F, int col, double* out,
+                                                  int check) {
+  ftran_col_body(M, I, F, col, out, check, blockIdx.x);
+}
+
 // Forward etas (linalg.cpp:21-32) on x0 = B0^{-1} a.  The multiplier of eta
 // k is the value at its pivot just before it is applied, s = MF (mask .* x0
 // at the pivots); every workgroup forms s from MF, then updates its slice:
 // x = x0 + sum_k s_k eta_k, with a pivot position overwritten by s_k eta_k[p_k].
 // mode 0: out[i] = x; mode 1: x_var[basis[i]] = x (dualSimplexInit's scatter).
-__global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* __restrict__ x0,
-                                                  double* out, int t, int mode, int check) {
+__device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter& I, const double* __restrict__ x0,
+                                               double* out, int t, int mode, int check, int bid) {
   DevScalars* S = I.S;
   if (skip_launch(S, check)) return;
   __shared__ double xm[ETA_MAX];
@@ -789,7 +807,7 @@ __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const d
     sk[k] = (a0 + a1) + (a2 + a3);
   }
   __syncthreads();
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = bid * BS + threadIdx.x;
   if (i >= M.m) return;
   double x = x0[i];
   // Eight eta entries are loaded ahead of their (order-dependent) application.
@@ -815,6 +833,11 @@ __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const d
   else out[I.basis[i]] = x;
 }
 
+__global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* __restrict__ x0,
+                                                  double* out, int t, int mode, int check) {
+  ftran_out_body(M, I, x0, out, t, mode, check, blockIdx.x);
+}
+
 // Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
 __global__ __launch_bounds__(BS) void k_gather(const double* src, const int* map, double* dst, int len,
                                                const DevScalars* S, int check) {
@@ -823,12 +846,11 @@ __global__ __launch_bounds__(BS) void k_gather(const double* src, const int* map
   if (i < len) dst[i] = src[map[i]];
 }
 
-__global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int rows, int cols,
-                                             const double* __restrict__ xin, const int* imap,
-                                             double* __restrict__ dst, const int* omap, const DevScalars* S,
-                                             int check) {
+__device__ __forceinline__ void gemv_body(const double* __restrict__ A, int rows, int cols,
+                                          const double* __restrict__ xin, const int* imap, double* __restrict__ dst,
+                                          const int* omap, const DevScalars* S, int check, int bid) {
   if (skip_launch(S, check)) return;
-  const int w = (blockIdx.x * BS + threadIdx.x) / WAVE;
+  const int w = (bid * BS + threadIdx.x) / WAVE;
   const int lane = threadIdx.x % WAVE;
   if (w >= rows) return;
   const double* row = A + (size_t)w * cols;
@@ -841,16 +863,59 @@ __global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int r
   if (lane == 0) dst[omap ? omap[w] : w] = acc;
 }
 
-__global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, const double* b, double* dst,
-                                                    const DevScalars* S, int check) {
+__global__ __launch_bounds__(BS) void k_gemv(const double* __restrict__ A, int rows, int cols,
+                                             const double* __restrict__ xin, const int* imap,
+                                             double* __restrict__ dst, const int* omap, const DevScalars* S,
+                                             int check) {
+  gemv_body(A, rows, cols, xin, imap, dst, omap, S, check, blockIdx.x);
+}
+
+__device__ __forceinline__ void ftran_slack_body(const DevModel& M, const DevFactor& F, const double* b, double* dst,
+                                                 const DevScalars* S, int check, int bid) {
   if (skip_launch(S, check)) return;
-  int r = blockIdx.x * BS + threadIdx.x;
+  int r = bid * BS + threadIdx.x;
   if (r >= M.m) return;
   int sp = F.slackpos_of_row[r];
   if (sp < 0) return;
   double s = 0.0;
   for (int q = F.crp[r]; q < F.crp[r + 1]; ++q) s += F.ccv[q] * dst[F.pos_P[F.cci[q]]];
   dst[sp] = b[r] - s;
+}
+
+__global__ __launch_bounds__(BS) void k_ftran_slack(DevModel M, DevFactor F, const double* b, double* dst,
+                                                    const DevScalars* S, int check) {
+  ftran_slack_body(M, F, b, dst, S, check, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Stages of one iteration that do not depend on each other, in one launch:
+// workgroups [0, nb1) run the first stage and the rest the second.  The DSE
+// vector's FTRAN (tau = B^{-1} rho) needs only rho, so its three steps ride
+// along with the pivot-row SpMV, CHUZC (textbook) and the entering column's
+// FTRAN: the chain of a phase-1 or phase-2 iteration without bound flips has
+// eight launches instead of eleven.  Each stage runs its kernel's code
+// unchanged (same summation orders); tau's pre-eta values go to w2, so the
+// entering column keeps x0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_stage_spmv_gemv(DevModel M, const int* __restrict__ posN,
+                                                        const double* __restrict__ rho,
+                                                        const double* __restrict__ cvec, double* __restrict__ alpha_r,
+                                                        const DevScalars* S, int cols, int nb1, DevFactor F,
+                                                        double* __restrict__ tau0) {
+  if ((int)blockIdx.x < nb1) spmv_body(M, posN, rho, 1.0, 0.0, cvec, alpha_r, 0, S, 1, cols, blockIdx.x);
+  else gemv_body(F.Kinv, F.k, F.k, rho, F.row_R, tau0, F.pos_P, S, 1, blockIdx.x - nb1);
+}
+
+__global__ __launch_bounds__(BS) void k_stage_chuzc_slack(DevModel M, DevIter I, int phase, int it, int nb1,
+                                                          DevFactor F, const double* rho, double* tau0) {
+  if ((int)blockIdx.x < nb1) chuzc_tb_body(M, I, phase, it, blockIdx.x, nb1);
+  else ftran_slack_body(M, F, rho, tau0, I.S, 1, blockIdx.x - nb1);
+}
+
+__global__ __launch_bounds__(BS) void k_stage_col_out(DevModel M, DevIter I, DevFactor F, double* colout, int nb1,
+                                                      const double* tau0, double* tau, int t) {
+  if ((int)blockIdx.x < nb1) ftran_col_body(M, I, F, -1, colout, 1, blockIdx.x);
+  else ftran_out_body(M, I, tau0, tau, t, 0, 1, blockIdx.x - nb1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1859,6 +1924,26 @@ void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double
                           beta, cvec, out, by_col, S, check, cols);
   else
     k_spmv<<<nblk(M.N, cols), SPMV_THREADS, 0, s>>>(M, posN, v, alpha, beta, cvec, out, by_col, S, check, cols);
+}
+void launch_stage_spmv_gemv(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const int* posN,
+                            const double* rho, const double* cvec, double* alpha_r, double* tau0) {
+  const int cols = spmv_cols(M);
+  const int nb1 = nblk(M.N, cols);
+  const int nb2 = F.k > 0 ? nblk((long)F.k * WAVE, BS) : 0;
+  k_stage_spmv_gemv<<<nb1 + nb2, BS, 0, s>>>(M, posN, rho, cvec, alpha_r, I.S, cols, nb1, F, tau0);
+}
+void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase,
+                              int it, const double* rho, double* tau0) {
+  const int nb1 = cap_grid(M.N - M.m, BS, I.part_cap);
+  k_stage_chuzc_slack<<<nb1 + nblk(M.m, BS), BS, 0, s>>>(M, I, phase, it, nb1, F, rho, tau0);
+}
+void launch_stage_col_out(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t,
+                          double* colout, const double* tau0, double* tau) {
+  const int nb1 = nblk(M.m, BS);
+  k_stage_col_out<<<nb1 + (t > 0 ? nblk(M.m, BS) : 0), BS, 0, s>>>(M, I, F, colout, nb1, tau0, tau, t);
+}
+void launch_ftran_eta(hipStream_t s, const DevModel& M, const DevIter& I, const double* x0, double* out, int t) {
+  if (t > 0) k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, x0, out, t, 0, 1);
 }
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);

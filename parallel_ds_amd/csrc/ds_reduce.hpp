@@ -95,17 +95,20 @@ __device__ inline double block_min(double v) {
 // invalidate their L2 (a sequentially consistent fence would, in every
 // workgroup, and the following loads of the launch and of the next kernels
 // would miss).
-__device__ inline bool last_block(int* ctr) {
+// nb: the number of workgroups that take part (all of the launch, or the
+// share of one stage in a launch that runs several stages side by side).
+__device__ inline bool last_block(int* ctr, int nb) {
   __shared__ int s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const int prev = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == (int)gridDim.x - 1);
+    s_last = (prev == nb - 1);
   }
   __syncthreads();
   if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_last != 0;
 }
+__device__ inline bool last_block(int* ctr) { return last_block(ctr, (int)gridDim.x); }
 
 }  // namespace pds

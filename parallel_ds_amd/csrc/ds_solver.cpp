@@ -740,6 +740,20 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   // execution interval (hipExtLaunchKernelGGL).
   // Every time_kernels-th iteration: a per-launch event pair costs the loop a few percent.
   const bool timed = P_.time_kernels > 0 && marked_ && it % P_.time_kernels == 0 && ev_used_ < (int)ev_iter_.size();
+  if (!timed && !P_.profile && P_.ratio_test == 0 && !has_boxed_) {
+    // The DSE FTRAN's stages share the launches of the SpMV, CHUZC and the
+    // entering column's FTRAN (same kernels' code, same results).
+    double* tau0 = t > 0 ? w2_.p : tau_.p;
+    launch_stage_spmv_gemv(st_, m, i, f, posN_.p, rho_.p, c_.p, alpha_r_.p, tau0);
+    launch_stage_chuzc_slack(st_, m, i, f, phase, it, rho_.p, tau0);
+    if (phase == 2 && force_due(DS_FORCE_STABILITY, it)) {
+      launch_force(st_, m, i, DS_FORCE_STABILITY);
+    }
+    launch_stage_col_out(st_, m, i, f, t, t > 0 ? x0_.p : alpha_q_.p, tau0, tau_.p);
+    launch_ftran_eta(st_, m, i, x0_.p, alpha_q_.p, t);
+    launch_dupdate_update(st_, m, i, phase, it);
+    return;
+  }
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1, timed ? ev_[2 * ev_used_] : nullptr,
               timed ? ev_[2 * ev_used_ + 1] : nullptr);
   if (timed) ev_iter_[ev_used_++] = it;
