@@ -66,6 +66,7 @@ struct DevScalars {
   int pad0;
   double delta, theta, obj, f_p, corr, alpha_p, alpha_rq, theta_P;
   double tmp0, tmp1;
+  double dq, wq;     // primal iterations: the entering column's reduced cost and Devex weight
   double dz;         // objective change of the flips' costs (first part of delta_z)
   int ctr[16];       // last-workgroup counters, reset by the finaliser
 };

@@ -1123,8 +1123,8 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
 // ratio test over the basic rows, BTRAN of the leaving row, the pivot-row SpMV
 // and the update of d, x_B and the eta file.  With d feasible the host
 // restores the bounds and the dual simplex finishes from that basis.  One
-// iteration: k_pprice -> FTRAN -> k_pratio1/2 -> BTRAN -> SpMV -> k_pprep ->
-// k_pupdate.  Models with Free or Boxed variables do not take this path.
+// iteration: k_pprice -> FTRAN -> k_pratio1/2 -> BTRAN -> SpMV -> k_pupdate.
+// Models with Free or Boxed variables do not take this path.
 // ---------------------------------------------------------------------------
 constexpr double PRIMAL_PIV = 1e-7;  // smallest |alpha_q[i]| of a leaving row
 constexpr double PRIMAL_TOL = 1e-9;  // feasibility slack of the first ratio pass
@@ -1166,6 +1166,8 @@ __global__ __launch_bounds__(BS) void k_pprice(DevModel M, DevIter I, int it) {
   S->q_idx = fi;
   S->q = q;
   S->neg = I.d[q] < 0 ? 0 : 1;
+  S->dq = I.d[q];
+  S->wq = I.devex[q];
 }
 
 // Ratio of basic row i for the step of x_q in direction dir: the row blocks
@@ -1255,28 +1257,20 @@ __global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it) {
   S->is_lower = a > 0 ? 1 : 0;
 }
 
-// After the pivot-row SpMV: alpha_r[q] for the stability check, the dual step
-// theta = d_q / alpha_q[p] (alpha_q[p] passed the pivot tolerance) and the
-// entering column's Devex weight.
-__global__ void k_pprep(DevModel M, DevIter I) {
-  DevScalars* S = I.S;
-  if (stopped(S) || threadIdx.x != 0) return;
-  S->alpha_rq = I.alpha_r[S->q_idx];
-  S->theta = I.d[S->q] / I.alpha_q[S->p_idx];
-  S->tmp1 = I.devex[S->q];
-}
-
 // d_N -= theta alpha_r with the Devex weights w_j = max(w_j, (alpha_j /
 // alpha_q)^2 w_q), then the basis change of k_dupdate_update without the DSE
 // weights (they are recomputed exactly when the dual simplex takes over).
+// Every workgroup forms, from values earlier launches wrote, the dual step
+// theta = d_q / alpha_q[p] (alpha_q[p] passed the pivot tolerance) and
+// alpha_r[q] for the stability check.
 __global__ __launch_bounds__(BS) void k_pupdate(DevModel M, DevIter I, int it) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
-  const double theta = S->theta;
-  const UpdateArgs u = update_args(M, I, 2);
-  const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
-  const double arq = S->alpha_rq, wq = S->tmp1;
   const int q_idx = S->q_idx;
+  const double arq = I.alpha_r[q_idx], wq = S->wq;
+  const double theta = S->dq / I.alpha_q[S->p_idx];
+  const UpdateArgs u = update_args(M, I, 2);
+  const bool stable = !(fabs(u.ap - arq) > EPS_R * (1 + fabs(u.ap)));
   const int nN = M.N - M.m;
   int dinf = 0;
   for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
@@ -1299,6 +1293,8 @@ __global__ __launch_bounds__(BS) void k_pupdate(DevModel M, DevIter I, int it) {
   td = block_sumi<BS>(td);
   if (threadIdx.x == 0) {
     S->ctr[CTR_DUPD] = 0;
+    S->alpha_rq = arq;
+    S->theta = theta;
     dupdate_final(M, I, 2, it, theta, td);
     I.devex[S->p] = fmax(wq / (arq * arq), 1.0);
   }
@@ -2001,7 +1997,6 @@ void launch_pratio(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_pratio2<<<g, BS, 0, s>>>(M, I, it);
 }
 void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
-  k_pprep<<<1, WAVE, 0, s>>>(M, I);
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   k_pupdate<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, it);
 }
