@@ -197,15 +197,21 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
           cs[b][wv] = crow;
         }
         __syncthreads();
+        // The waves' candidates: every LDS load first, then the comparisons.
+        double cvq[TS_WAVES];
+        int ciq[TS_WAVES];
+#pragma unroll
+        for (int q = 0; q < TS_WAVES; ++q) {
+          cvq[q] = cv[b][q];
+          ciq[q] = ci[b][q];
+        }
         double gv = -2.0;
         int gr = INT_MAX;
 #pragma unroll
         for (int q = 0; q < TS_WAVES; ++q) {
-          const double v2 = cv[b][q];
-          const int r2 = ci[b][q];
-          const bool better = v2 > gv || (v2 == gv && r2 < gr);
-          gv = better ? v2 : gv;
-          gr = better ? r2 : gr;
+          const bool better = cvq[q] > gv || (cvq[q] == gv && ciq[q] < gr);
+          gv = better ? cvq[q] : gv;
+          gr = better ? ciq[q] : gr;
         }
         gv = uniform_d(gv);
         gr = uniform_i(gr);
@@ -225,8 +231,22 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
         if (gv > 0.0 && !dep) {
           const double* pr = cand[b][gr / WAVE];
           const double l = a[s] * recip_nr(pr[s]);
+          // The winner's row in register batches of NB / 4: each batch's LDS
+          // loads issue back to back before its FMAs, instead of one or two
+          // ahead of them.  Columns at or before the pivot column in a batch
+          // are updated too: they were eliminated earlier in this block, are
+          // never read again, and their LDS entries hold finite rows of
+          // earlier steps.
+          constexpr int H = NB / 4;
 #pragma unroll
-          for (int x = s + 1; x < NB; ++x) a[x] = a[x] - l * pr[x];
+          for (int h = 0; h < NB / H; ++h) {
+            if ((h + 1) * H <= s + 1) continue;
+            double pv[H];
+#pragma unroll
+            for (int y = 0; y < H; ++y) pv[y] = pr[h * H + y];
+#pragma unroll
+            for (int y = 0; y < H; ++y) a[h * H + y] = a[h * H + y] - l * pv[y];
+          }
         }
       }
     }
