@@ -154,7 +154,37 @@ __global__ __launch_bounds__(BS) void k_dualcheck(DevModel M, DevIter I, int it)
 // one parallel matvec (b_k = eta_k[p] while position p still holds its unit
 // entry).  The kernel-inverse gather then runs on all rows in k_btran_out.
 // ---------------------------------------------------------------------------
-constexpr int MF_TILE = 64;  // eta counts up to this keep MF's lower triangle in LDS
+// MF's lower triangle, packed by rows (row r from r(r+1)/2 on), goes to
+// dynamic LDS when the eta count is at most MF_TRI_MAX (125 KB at 176).
+constexpr int MF_TRI_MAX = 176;
+constexpr int MF_SEQ_MAX = 64;  // up to this eta count BTRAN sums MF's columns in one chain, beyond in four
+__host__ __device__ constexpr int tri(int r) { return r * (r + 1) / 2; }
+static size_t mf_lds_bytes(int t) { return t > 0 && t <= MF_TRI_MAX ? sizeof(double) * tri(t) : 0; }
+
+// Every thread of the workgroup: rows 0..t-1 of MF's lower triangle into mft,
+// eight independent loads in flight per thread.
+__device__ __forceinline__ void load_mf_tri(const DevIter& I, int t, double* mft) {
+  constexpr int U = 8;
+  const int n = tri(t), cap = I.eta_cap;
+  for (int e0 = threadIdx.x; e0 < n; e0 += BS * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * BS;
+      v[u] = 0.0;
+      if (e < n) {
+        int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+        while (tri(r + 1) <= e) ++r;
+        while (tri(r) > e) --r;
+        v[u] = I.MF[(size_t)r * cap + (e - tri(r))];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * BS < n) mft[e0 + u * BS] = v[u];
+  }
+}
+
 __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed,
                                                    int cand) {
   DevScalars* S = I.S;
@@ -173,22 +203,18 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   __shared__ int pidx[ETA_MAX + 1];  // kernel column of the position, -1 at a slack position
   __shared__ int srow[ETA_MAX + 1];  // slack row of a slack position
   __shared__ int ccb[ETA_MAX + 1], cce[ETA_MAX + 1];  // its coupling-row extent
-  __shared__ double mf[MF_TILE * MF_TILE];
+  extern __shared__ double mft[];  // MF's lower triangle (t <= MF_TRI_MAX)
   __shared__ int kmax_p, kmin_p;
   const int ns = t + 1;
   const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
-  const bool tile = t <= MF_TILE;
+  const bool tile = t <= MF_TRI_MAX;
   if (tid == 0) {
     kmax_p = -1;
     kmin_p = INT_MAX;
   }
   for (int s = tid; s < ns; s += BS) pos[s] = (s == 0) ? p : I.eta_pos[s - 1];
   // MF's lower triangle (the part the sweep reads) into LDS when it fits.
-  if (tile)
-    for (int e = tid; e < t * MF_TILE; e += BS) {
-      const int r = e / MF_TILE, c = e % MF_TILE;
-      mf[e] = (c <= r) ? I.MF[(size_t)r * cap + c] : 0.0;
-    }
+  if (tile) load_mf_tri(I, t, mft);
   __syncthreads();
   for (int k = tid; k < t; k += BS)
     if (pos[1 + k] == p) {
@@ -224,17 +250,26 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
   __syncthreads();
   for (int k = tid; k < t; k += BS) {
     double acc = 0.0;
-    if (tile) {
-      for (int k2 = k; k2 < t; ++k2) acc += mf[k2 * MF_TILE + k] * bv[k2];
+    if (t <= MF_SEQ_MAX) {
+      for (int k2 = k; k2 < t; ++k2) acc += mft[tri(k2) + k] * bv[k2];
     } else {
-      // four independent partial sums keep four MF loads in flight
+      // four independent partial sums (from LDS or, beyond MF_TRI_MAX, with
+      // four MF loads in flight)
       double a4[4] = {0.0, 0.0, 0.0, 0.0};
       int k2 = k;
-      for (; k2 + 3 < t; k2 += 4) {
+      if (tile) {
+        for (; k2 + 3 < t; k2 += 4) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a4[q] += I.MF[(size_t)(k2 + q) * cap + k] * bv[k2 + q];
+          for (int q = 0; q < 4; ++q) a4[q] += mft[tri(k2 + q) + k] * bv[k2 + q];
+        }
+        for (; k2 < t; ++k2) a4[0] += mft[tri(k2) + k] * bv[k2];
+      } else {
+        for (; k2 + 3 < t; k2 += 4) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a4[q] += I.MF[(size_t)(k2 + q) * cap + k] * bv[k2 + q];
+        }
+        for (; k2 < t; ++k2) a4[0] += I.MF[(size_t)k2 * cap + k] * bv[k2];
       }
-      for (; k2 < t; ++k2) a4[0] += I.MF[(size_t)k2 * cap + k] * bv[k2];
       acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
     }
     u[k] = acc;
@@ -782,43 +817,58 @@ __device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter&
   __shared__ double xm[ETA_MAX];
   __shared__ double sk[ETA_MAX];
   __shared__ int pk[ETA_MAX];
+  extern __shared__ double mft[];  // MF's lower triangle (t <= MF_TRI_MAX)
   const int cap = I.eta_cap;
+  const bool tile = t <= MF_TRI_MAX;
   for (int k = threadIdx.x; k < t; k += BS) {
     int q = I.eta_pos[k];
     pk[k] = q;
     xm[k] = (I.eta_last[k] < 0) ? x0[q] : 0.0;
   }
+  if (tile) load_mf_tri(I, t, mft);
   __syncthreads();
-  // s = MF xm: one thread per row of MF, its row read in 16-byte pairs with
-  // four independent partial sums, so that many loads are in flight per thread.
+  // s = MF xm: one thread per row of MF with four independent partial sums;
+  // the row comes from LDS, or from global memory in 16-byte pairs.
   for (int k = threadIdx.x; k < t; k += BS) {
-    const double* row = I.MF + (size_t)k * cap;  // 16-byte aligned: cap is even
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int j = 0;
-    for (; j + 3 <= k; j += 4) {
-      const double2 r0 = *reinterpret_cast<const double2*>(row + j);
-      const double2 r1 = *reinterpret_cast<const double2*>(row + j + 2);
-      a0 += r0.x * xm[j];
-      a1 += r0.y * xm[j + 1];
-      a2 += r1.x * xm[j + 2];
-      a3 += r1.y * xm[j + 3];
+    if (tile) {
+      const double* row = mft + tri(k);
+      for (; j + 3 <= k; j += 4) {
+        a0 += row[j] * xm[j];
+        a1 += row[j + 1] * xm[j + 1];
+        a2 += row[j + 2] * xm[j + 2];
+        a3 += row[j + 3] * xm[j + 3];
+      }
+      for (; j <= k; ++j) a0 += row[j] * xm[j];
+    } else {
+      const double* row = I.MF + (size_t)k * cap;  // 16-byte aligned: cap is even
+      for (; j + 3 <= k; j += 4) {
+        const double2 r0 = *reinterpret_cast<const double2*>(row + j);
+        const double2 r1 = *reinterpret_cast<const double2*>(row + j + 2);
+        a0 += r0.x * xm[j];
+        a1 += r0.y * xm[j + 1];
+        a2 += r1.x * xm[j + 2];
+        a3 += r1.y * xm[j + 3];
+      }
+      for (; j <= k; ++j) a0 += row[j] * xm[j];
     }
-    for (; j <= k; ++j) a0 += row[j] * xm[j];
     sk[k] = (a0 + a1) + (a2 + a3);
   }
   __syncthreads();
   const int i = bid * BS + threadIdx.x;
   if (i >= M.m) return;
   double x = x0[i];
-  // Eight eta entries are loaded ahead of their (order-dependent) application.
+  // Sixteen eta entries are loaded ahead of their (order-dependent) application.
+  constexpr int EA = 16;
   const double* ei = I.etas + i;
   int k = 0;
-  for (; k + 8 <= t; k += 8) {
-    double e[8];
+  for (; k + EA <= t; k += EA) {
+    double e[EA];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) e[q] = ei[(size_t)(k + q) * M.m];
+    for (int q = 0; q < EA; ++q) e[q] = ei[(size_t)(k + q) * M.m];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < EA; ++q) {
       const double s = sk[k + q];
       if (pk[k + q] == i) x = s * e[q];
       else if (s != 0.0) x = x + s * e[q];
@@ -1904,11 +1954,11 @@ void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it
 }
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed) {
-  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, p_fixed, -1);
+  k_btran_prep<<<1, BS, mf_lds_bytes(t), s>>>(M, I, F, t, p_fixed, -1);
   k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, out, p_fixed < 0 ? 1 : 0, -1);
 }
 void launch_btran_cand(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, int cand) {
-  k_btran_prep<<<1, BS, 0, s>>>(M, I, F, t, -1, cand);
+  k_btran_prep<<<1, BS, mf_lds_bytes(t), s>>>(M, I, F, t, -1, cand);
   k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, I.rho_c + (size_t)cand * M.m, 1, cand);
 }
 void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double* v, double alpha, double beta,
@@ -1936,10 +1986,11 @@ void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I
 void launch_stage_col_out(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t,
                           double* colout, const double* tau0, double* tau) {
   const int nb1 = nblk(M.m, BS);
-  k_stage_col_out<<<nb1 + (t > 0 ? nblk(M.m, BS) : 0), BS, 0, s>>>(M, I, F, colout, nb1, tau0, tau, t);
+  k_stage_col_out<<<nb1 + (t > 0 ? nblk(M.m, BS) : 0), BS, mf_lds_bytes(t), s>>>(M, I, F, colout, nb1, tau0, tau,
+                                                                                  t);
 }
 void launch_ftran_eta(hipStream_t s, const DevModel& M, const DevIter& I, const double* x0, double* out, int t) {
-  if (t > 0) k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, x0, out, t, 0, 1);
+  if (t > 0) k_ftran_out<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, x0, out, t, 0, 1);
 }
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);
@@ -1954,7 +2005,7 @@ void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const 
                       double* out, int check) {
   if (t > 0) {
     k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, I.x0, check);
-    k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, I.x0, out, t, 0, check);
+    k_ftran_out<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, I.x0, out, t, 0, check);
   } else {
     k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, out, check);
   }
@@ -1966,7 +2017,8 @@ void launch_ftran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
   if (F.k > 0)
     k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.Kinv, F.k, F.k, b, F.row_R, tmp, F.pos_P, I.S, check);
   k_ftran_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, b, tmp, I.S, check);
-  if (t > 0 || mode == 1) k_ftran_out<<<nblk(M.m, BS), BS, 0, s>>>(M, I, tmp, out, t, mode, check);
+  if (t > 0 || mode == 1)
+    k_ftran_out<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, tmp, out, t, mode, check);
 }
 void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, double* v,
                         double* y, int t) {
