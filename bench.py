@@ -602,7 +602,8 @@ def main():
     ap.add_argument("--steps", type=int, default=8, help="held-out family LPs timed (per start)")
     ap.add_argument("--warmup", type=int, default=1, help="held-out family LPs solved untimed first")
     ap.add_argument("--train", type=int, default=16, help="GPU-solved training instances of the family")
-    ap.add_argument("--threads", type=int, default=4, help="family LPs solved at a time (contexts/streams)")
+    ap.add_argument("--threads", type=int, default=20,
+                    help="family LPs solved at a time (contexts/streams, one hardware queue each)")
     ap.add_argument("--family-refactor", type=int, default=150)
     ap.add_argument("--epochs", type=int, default=48, help="training epochs of the family network (minibatch 2048)")
     ap.add_argument("--family-cpu-iters", type=int, default=300)
@@ -639,6 +640,9 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
+    # One hardware queue per concurrent family solve (HIP maps streams onto 4 by default); set
+    # before the first HIP call of the process, within the runtime's limit of 32.
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(32, args.threads))))
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)
@@ -667,7 +671,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 LP generator)",
         "config": {"workload": fam["workload"], "m": 2000, "n": 5000, "nnz_per_col": 5, "sigma": 0.1,
                    "train_instances": args.train, "held_out_timed": args.steps, "held_out_untimed": args.warmup,
-                   "concurrent_solves": args.threads, "refactor_freq": args.family_refactor,
+                   "concurrent_solves": args.threads, "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                   "refactor_freq": args.family_refactor,
                    "slack_solves_per_s": fam["slack"]["solves_per_s"], "speedup": fam["speedup_solves_per_s"]},
         "roofline": single["roofline"] if single else None,
         "cpu_baseline": fam.get("cpu_baseline"),
