@@ -640,9 +640,11 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
-    # One hardware queue per concurrent family solve (HIP maps streams onto 4 by default); set
+    # One hardware queue per concurrent family solve (HIP maps streams onto 4 by default): raised
     # before the first HIP call of the process, within the runtime's limit of 32.
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(32, args.threads))))
+    want = max(4, min(32, args.threads))
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < want:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)

@@ -34,8 +34,15 @@ __device__ inline void raise_status(DevScalars* S, int st, int it) {
 // (sequentialDualSimplex.cpp:305-310) and the phase-1 termination tests
 // (tpp:841-875).  Argmax of delta^2/beta, ties to the lowest position.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void btran_prep_body(const DevModel& M, const DevIter& I, const DevFactor& F, int t,
+                                                int p_fixed, int cand);
+
+// BTRAN = true: the last workgroup goes on with BTRAN's set-up for the chosen
+// row (k_btran_prep's work, one workgroup), so the iteration needs one launch
+// fewer; the launch then carries BTRAN's dynamic LDS.
+template <bool BTRAN>
 __global__ __launch_bounds__(BS) void k_price(DevModel M, DevIter I, int phase, int it, int skip_top,
-                                              int perturb_on) {
+                                              int perturb_on, DevFactor F, int t) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
   if (!skip_top) {
@@ -85,41 +92,50 @@ __global__ __launch_bounds__(BS) void k_price(DevModel M, DevIter I, int phase, 
   }
   block_argmax<BS>(fw, fi);
   fp = block_sumi<BS>(fp);
-  if (threadIdx.x != 0) return;
-  S->ctr[CTR_PRICE] = 0;
-  bool found = fi != INT_MAX;
-  if (phase == 2) {
-    if (!fp) {
-      raise_status(S, ST_OPTIMAL, it);
-    } else if (!found) {
-      raise_status(S, ST_NO_PIVOT, it);
+  __shared__ int s_row;  // the chosen row, or -1 when an event was raised
+  if (threadIdx.x == 0) {
+    s_row = -1;
+    S->ctr[CTR_PRICE] = 0;
+    bool found = fi != INT_MAX;
+    if (phase == 2) {
+      if (!fp) {
+        raise_status(S, ST_OPTIMAL, it);
+      } else if (!found) {
+        raise_status(S, ST_NO_PIVOT, it);
+      } else {
+        int j = I.basis[fi];
+        double dl;
+        var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl);
+        S->p_idx = fi;
+        S->p = j;
+        S->delta = dl;
+        S->is_lower = dl < 0 ? 1 : 0;
+        S->neg = S->is_lower;
+        s_row = fi;
+      }
     } else {
-      int j = I.basis[fi];
-      double dl;
-      var_infeas(M.bt[j], I.x[j], M.l[j], M.u[j], dl);
-      S->p_idx = fi;
-      S->p = j;
-      S->delta = dl;
-      S->is_lower = dl < 0 ? 1 : 0;
-      S->neg = S->is_lower;
+      int dinf = S->dinf;
+      double obj = S->obj;
+      int st = ST_RUNNING;
+      if (!found || dinf == 0) {
+        if (fabs(obj) < EPS_A || dinf == 0) st = ST_PH1_SOLVED;
+        else if (obj > 0) st = ST_PH1_DUALINF;
+      }
+      if (st == ST_RUNNING && (!found || obj > EPS_BOUND)) st = ST_PH1_RESTART;
+      if (st != ST_RUNNING) {
+        raise_status(S, st, it);
+      } else {
+        S->p_idx = fi;
+        S->p = I.basis[fi];
+        S->f_p = I.f[fi];
+        S->neg = I.f[fi] > 0 ? 1 : 0;
+        s_row = fi;
+      }
     }
-  } else {
-    int dinf = S->dinf;
-    double obj = S->obj;
-    int st = ST_RUNNING;
-    if (!found || dinf == 0) {
-      if (fabs(obj) < EPS_A || dinf == 0) st = ST_PH1_SOLVED;
-      else if (obj > 0) st = ST_PH1_DUALINF;
-    }
-    if (st == ST_RUNNING && (!found || obj > EPS_BOUND)) st = ST_PH1_RESTART;
-    if (st != ST_RUNNING) {
-      raise_status(S, st, it);
-    } else {
-      S->p_idx = fi;
-      S->p = I.basis[fi];
-      S->f_p = I.f[fi];
-      S->neg = I.f[fi] > 0 ? 1 : 0;
-    }
+  }
+  if constexpr (BTRAN) {
+    __syncthreads();
+    if (s_row >= 0) btran_prep_body(M, I, F, t, s_row, -1);
   }
 }
 
@@ -185,8 +201,8 @@ __device__ __forceinline__ void load_mf_tri(const DevIter& I, int t, double* mft
   }
 }
 
-__global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed,
-                                                   int cand) {
+__device__ __forceinline__ void btran_prep_body(const DevModel& M, const DevIter& I, const DevFactor& F, int t,
+                                                int p_fixed, int cand) {
   DevScalars* S = I.S;
   if (p_fixed < 0 && stopped(S)) return;
   if (cand >= 0 && cand >= S->ncand) return;
@@ -352,6 +368,11 @@ __global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFac
       I.list_n[1] = a;
     }
   }
+}
+
+__global__ __launch_bounds__(BS) void k_btran_prep(DevModel M, DevIter I, DevFactor F, int t, int p_fixed,
+                                                   int cand) {
+  btran_prep_body(M, I, F, t, p_fixed, cand);
 }
 
 // A workgroup takes BO_ROWS consecutive rows; its BS / BO_ROWS row groups
@@ -1271,7 +1292,10 @@ __global__ __launch_bounds__(BS) void k_pratio1(DevModel M, DevIter I, int it) {
 // Pass 2: among blocking rows whose exact ratio is within theta_max, the
 // largest |alpha_q[i]| (ties to the lowest position) leaves; delta = x_p -
 // its bound, so the basis change moves x_q by delta / alpha_q[p].
-__global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it) {
+// BTRAN = true: the last workgroup goes on with BTRAN's set-up for the leaving
+// row, as k_price<true> does.
+template <bool BTRAN>
+__global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it, DevFactor F, int t) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
   const double dir = S->neg ? -1.0 : 1.0;
@@ -1293,18 +1317,26 @@ __global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it) {
   int fi = INT_MAX;
   for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fb, fi, I.part_d[b], I.part_i[b]);
   block_argmax<BS>(fb, fi);
-  if (threadIdx.x != 0) return;
-  S->ctr[CTR_CHUZC2] = 0;
-  if (fi == INT_MAX) {
-    raise_status(S, ST_NO_PIVOT, it);
-    return;
+  __shared__ int s_row;  // the leaving row, or -1 when an event was raised
+  if (threadIdx.x == 0) {
+    s_row = -1;
+    S->ctr[CTR_CHUZC2] = 0;
+    if (fi == INT_MAX) {
+      raise_status(S, ST_NO_PIVOT, it);
+    } else {
+      double a, x, bd;
+      primal_row(M, I, dir, fi, a, x, bd);
+      S->p_idx = fi;
+      S->p = I.basis[fi];
+      S->delta = x - bd;
+      S->is_lower = a > 0 ? 1 : 0;
+      s_row = fi;
+    }
   }
-  double a, x, bd;
-  primal_row(M, I, dir, fi, a, x, bd);
-  S->p_idx = fi;
-  S->p = I.basis[fi];
-  S->delta = x - bd;
-  S->is_lower = a > 0 ? 1 : 0;
+  if constexpr (BTRAN) {
+    __syncthreads();
+    if (s_row >= 0) btran_prep_body(M, I, F, t, s_row, -1);
+  }
 }
 
 // d_N -= theta alpha_r with the Devex weights w_j = max(w_j, (alpha_j /
@@ -1947,7 +1979,13 @@ static inline int cap_grid(long n, int bs, int cap) {
 }
 
 void launch_price(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it, int skip_top, int perturb_on) {
-  k_price<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it, skip_top, perturb_on);
+  k_price<false><<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it, skip_top, perturb_on, DevFactor{}, 0);
+}
+void launch_price_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
+                        int skip_top, int perturb_on, int t, double* rho) {
+  k_price<true><<<cap_grid(M.m, BS, I.part_cap), BS, mf_lds_bytes(t), s>>>(M, I, phase, it, skip_top, perturb_on, F,
+                                                                         t);
+  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, rho, 1, -1);
 }
 void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_dualcheck<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
@@ -2043,10 +2081,12 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
 void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_pprice<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
 }
-void launch_pratio(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
+void launch_pratio_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int it, int t,
+                         double* rho) {
   const int g = cap_grid(M.m, BS, I.part_cap);
   k_pratio1<<<g, BS, 0, s>>>(M, I, it);
-  k_pratio2<<<g, BS, 0, s>>>(M, I, it);
+  k_pratio2<true><<<g, BS, mf_lds_bytes(t), s>>>(M, I, it, F, t);
+  k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, rho, 1, -1);
 }
 void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;

@@ -8,6 +8,10 @@ namespace pds {
 constexpr int ETA_MAX = 256;  // eta file capacity (refactor interval + margin)
 
 void launch_price(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it, int skip_top, int perturb_on);
+// CHUZR whose last workgroup goes on with BTRAN's set-up, then BTRAN's output
+// (rho): launch_price + launch_btran_unit in two launches instead of three.
+void launch_price_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
+                        int skip_top, int perturb_on, int t, double* rho);
 void launch_dualcheck(hipStream_t s, const DevModel& M, const DevIter& I, int it);
 void launch_btran_unit(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t, double* out,
                        int p_fixed);
@@ -40,7 +44,10 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
 // two-pass ratio test over the basic rows, and the update (after FTRAN, BTRAN
 // and the pivot-row SpMV).
 void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it);
-void launch_pratio(hipStream_t s, const DevModel& M, const DevIter& I, int it);
+// The primal ratio test's two passes, the second one going on with BTRAN's
+// set-up for the leaving row, then BTRAN's output (rho).
+void launch_pratio_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int it, int t,
+                         double* rho);
 void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it);
 // Bound-flipping ratio test: the flip loop after CHUZC, then the flip set, the
 // column change, its FTRAN and the x_B / objective correction.

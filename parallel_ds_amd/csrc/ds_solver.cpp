@@ -732,17 +732,15 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
     if (force_due(DS_FORCE_DUAL_RESTART, it)) launch_force(st_, m, i, DS_FORCE_DUAL_RESTART);
     launch_dualcheck(st_, m, i, it);
   }
-  launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
-  prof_event(DS_PART_PRICING);
-  launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
-  prof_event(DS_PART_BTRAN);
   // Timed launches carry a start/stop event pair that records the kernel's own
   // execution interval (hipExtLaunchKernelGGL).
   // Every time_kernels-th iteration: a per-launch event pair costs the loop a few percent.
   const bool timed = P_.time_kernels > 0 && marked_ && it % P_.time_kernels == 0 && ev_used_ < (int)ev_iter_.size();
   if (!timed && !P_.profile && P_.ratio_test == 0 && !has_boxed_) {
-    // The DSE FTRAN's stages share the launches of the SpMV, CHUZC and the
-    // entering column's FTRAN (same kernels' code, same results).
+    // CHUZR goes on with BTRAN's set-up, and the DSE FTRAN's stages share the
+    // launches of the SpMV, CHUZC and the entering column's FTRAN (same
+    // kernels' code, same results): seven launches per iteration.
+    launch_price_btran(st_, m, i, f, phase, it, skip_top, P_.perturb, t, rho_.p);
     double* tau0 = t > 0 ? w2_.p : tau_.p;
     launch_stage_spmv_gemv(st_, m, i, f, posN_.p, rho_.p, c_.p, alpha_r_.p, tau0);
     launch_stage_chuzc_slack(st_, m, i, f, phase, it, rho_.p, tau0);
@@ -754,6 +752,10 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
     launch_dupdate_update(st_, m, i, phase, it);
     return;
   }
+  launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
+  prof_event(DS_PART_PRICING);
+  launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  prof_event(DS_PART_BTRAN);
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1, timed ? ev_[2 * ev_used_] : nullptr,
               timed ? ev_[2 * ev_used_ + 1] : nullptr);
   if (timed) ev_iter_[ev_used_++] = it;
@@ -1193,8 +1195,7 @@ void Solver::enqueue_primal(int it, int t) {
   const DevFactor f = F();
   launch_pprice(st_, m, i, it);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
-  launch_pratio(st_, m, i, it);
-  launch_btran_unit(st_, m, i, f, t, rho_.p, -1);
+  launch_pratio_btran(st_, m, i, f, it, t, rho_.p);
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
   launch_pupdate(st_, m, i, it);
 }
