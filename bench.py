@@ -604,6 +604,8 @@ def main():
     ap.add_argument("--train", type=int, default=16, help="GPU-solved training instances of the family")
     ap.add_argument("--threads", type=int, default=20,
                     help="family LPs solved at a time (contexts/streams, one hardware queue each)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: one per concurrent family solve)")
     ap.add_argument("--family-refactor", type=int, default=150)
     ap.add_argument("--epochs", type=int, default=48, help="training epochs of the family network (minibatch 2048)")
     ap.add_argument("--family-cpu-iters", type=int, default=300)
@@ -640,11 +642,12 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
-    # One hardware queue per concurrent family solve (HIP maps streams onto 4 by default): raised
-    # before the first HIP call of the process, within the runtime's limit of 32.
-    want = max(4, min(32, args.threads))
-    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < want:
-        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
+    # Hardware queues the family solves' streams are spread over, set before the first HIP call
+    # of the process (HIP's default is 4; at most 32): by default one per concurrent solve.
+    # Measured at the family size with 20 solves at a time: 20 queues 1.97 / 0.91 solves/s
+    # (learned / slack), 4 queues 1.73 / 0.80, 5 queues 1.00 / 0.66.
+    q = args.hw_queues if args.hw_queues > 0 else max(4, args.threads)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, q))
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1).", file=sys.stderr)

@@ -826,19 +826,12 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
   ftran_col_body(M, I, F, col, out, check, blockIdx.x);
 }
 
-// Forward etas (linalg.cpp:21-32) on x0 = B0^{-1} a.  The multiplier of eta
-// k is the value at its pivot just before it is applied, s = MF (mask .* x0
-// at the pivots); every workgroup forms s from MF, then updates its slice:
-// x = x0 + sum_k s_k eta_k, with a pivot position overwritten by s_k eta_k[p_k].
-// mode 0: out[i] = x; mode 1: x_var[basis[i]] = x (dualSimplexInit's scatter).
-__device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter& I, const double* __restrict__ x0,
-                                               double* out, int t, int mode, int check, int bid) {
-  DevScalars* S = I.S;
-  if (skip_launch(S, check)) return;
-  __shared__ double xm[ETA_MAX];
-  __shared__ double sk[ETA_MAX];
-  __shared__ int pk[ETA_MAX];
-  extern __shared__ double mft[];  // MF's lower triangle (t <= MF_TRI_MAX)
+// The eta file's coefficients for an FTRAN whose eta-free part is x0:
+// xm[k] = x0 at eta k's pivot when no earlier eta has that pivot (else 0),
+// sk = MF xm, pk[k] = eta k's pivot.  Every thread of the workgroup takes part;
+// ends with a barrier.
+__device__ __forceinline__ void eta_coeffs(const DevIter& I, const double* __restrict__ x0, int t, double* xm,
+                                           double* sk, int* pk, double* mft) {
   const int cap = I.eta_cap;
   const bool tile = t <= MF_TRI_MAX;
   for (int k = threadIdx.x; k < t; k += BS) {
@@ -877,29 +870,52 @@ __device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter&
     sk[k] = (a0 + a1) + (a2 + a3);
   }
   __syncthreads();
-  const int i = bid * BS + threadIdx.x;
-  if (i >= M.m) return;
-  double x = x0[i];
+}
+
+// Row i of the FTRAN result: x0[i] with the etas applied in order (an eta on
+// row i replaces the value, the others add to it).  The entries of eta k at
+// row i are read from stride-m storage (`e`, eta k at e[k * ld]).
+__device__ __forceinline__ double eta_apply(double x, const double* __restrict__ e, size_t ld, const double* sk,
+                                            const int* pk, int i, int t) {
   // Sixteen eta entries are loaded ahead of their (order-dependent) application.
   constexpr int EA = 16;
-  const double* ei = I.etas + i;
   int k = 0;
   for (; k + EA <= t; k += EA) {
-    double e[EA];
+    double v[EA];
 #pragma unroll
-    for (int q = 0; q < EA; ++q) e[q] = ei[(size_t)(k + q) * M.m];
+    for (int q = 0; q < EA; ++q) v[q] = e[(size_t)(k + q) * ld];
 #pragma unroll
     for (int q = 0; q < EA; ++q) {
       const double s = sk[k + q];
-      if (pk[k + q] == i) x = s * e[q];
-      else if (s != 0.0) x = x + s * e[q];
+      if (pk[k + q] == i) x = s * v[q];
+      else if (s != 0.0) x = x + s * v[q];
     }
   }
   for (; k < t; ++k) {
     const double s = sk[k];
-    if (pk[k] == i) x = s * ei[(size_t)k * M.m];
-    else if (s != 0.0) x = x + s * ei[(size_t)k * M.m];
+    if (pk[k] == i) x = s * e[(size_t)k * ld];
+    else if (s != 0.0) x = x + s * e[(size_t)k * ld];
   }
+  return x;
+}
+
+// Forward etas (linalg.cpp:21-32) on x0 = B0^{-1} a.  The multiplier of eta
+// k is the value at its pivot just before it is applied, s = MF (mask .* x0
+// at the pivots); every workgroup forms s from MF, then updates its slice:
+// x = x0 + sum_k s_k eta_k, with a pivot position overwritten by s_k eta_k[p_k].
+// mode 0: out[i] = x; mode 1: x_var[basis[i]] = x (dualSimplexInit's scatter).
+__device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter& I, const double* __restrict__ x0,
+                                               double* out, int t, int mode, int check, int bid) {
+  DevScalars* S = I.S;
+  if (skip_launch(S, check)) return;
+  __shared__ double xm[ETA_MAX];
+  __shared__ double sk[ETA_MAX];
+  __shared__ int pk[ETA_MAX];
+  extern __shared__ double mft[];  // MF's lower triangle (t <= MF_TRI_MAX)
+  eta_coeffs(I, x0, t, xm, sk, pk, mft);
+  const int i = bid * BS + threadIdx.x;
+  if (i >= M.m) return;
+  const double x = eta_apply(x0[i], I.etas + i, M.m, sk, pk, i, t);
   if (mode == 0) out[i] = x;
   else out[I.basis[i]] = x;
 }
@@ -1008,14 +1024,14 @@ __device__ __forceinline__ void dupdate_body(const DevModel& M, const DevIter& I
 
 // Thread 0 of the last workgroup: entering/leaving duals, the stability check
 // of alpha_q[p] against alpha_r[q], bound flips, objective.
+// aqp: alpha_q at the leaving row.
 __device__ __forceinline__ void dupdate_final(const DevModel& M, const DevIter& I, int phase, int it, double theta,
-                                              int td) {
+                                              int td, double aqp) {
   DevScalars* S = I.S;
   const int p = S->p, q = S->q;
   I.d[p] = -theta;
   I.d[q] = 0.0;
   S->dinf = td + dual_infeas(M.bt[p], -theta);
-  const double aqp = I.alpha_q[S->p_idx];
   if (fabs(aqp - S->alpha_rq) > EPS_R * (1 + fabs(aqp))) {
     S->refac_mid = 1;
     S->stop_iter = it;
@@ -1042,7 +1058,7 @@ __global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase
   td = block_sumi<BS>(td);
   if (threadIdx.x != 0) return;
   S->ctr[CTR_DUPD] = 0;
-  dupdate_final(M, I, phase, it, theta, td);
+  dupdate_final(M, I, phase, it, theta, td, I.alpha_q[S->p_idx]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1092,15 +1108,20 @@ __device__ __forceinline__ void update_body(const DevModel& M, const DevIter& I,
 // it: the value update_body writes, recomputed from alpha_q (written by an
 // earlier launch) so that the last workgroup never reads eta entries that
 // other workgroups of the same launch wrote.
+__device__ __forceinline__ double eta_value(const UpdateArgs& u, int i, double ai) {
+  return (i != u.p_idx) ? -ai / u.ap : 1 / u.ap;
+}
 __device__ __forceinline__ double eta_entry(const DevIter& I, const UpdateArgs& u, int i) {
-  return (i != u.p_idx) ? -I.alpha_q[i] / u.ap : 1 / u.ap;
+  return eta_value(u, i, i != u.p_idx ? I.alpha_q[i] : 0.0);
 }
 
 // Last workgroup (every thread): new E entries, row t (eta_t at the older
 // pivots) and column t (older etas at the new pivot); then row t of
 // MF = (I - L)^{-1}, where L[t][i] = E[i][t] for i at or after the previous
 // eta on the same position; then (thread 0) the eta bookkeeping and the swap.
-__device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u) {
+// aq_pos (optional): alpha_q at the older etas' pivots, aq_pos[j] at eta j's.
+__device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u,
+                                             const double* aq_pos = nullptr) {
   DevScalars* S = I.S;
   const int t = u.t, p_idx = u.p_idx;
   const int cap = I.eta_cap;
@@ -1109,7 +1130,8 @@ __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I
   if (threadIdx.x == 0) s_last = -1;
   __syncthreads();
   for (int jj = threadIdx.x; jj < t; jj += BS) {
-    I.E[(size_t)t * cap + jj] = eta_entry(I, u, I.eta_pos[jj]);
+    I.E[(size_t)t * cap + jj] =
+        aq_pos ? eta_value(u, I.eta_pos[jj], aq_pos[jj]) : eta_entry(I, u, I.eta_pos[jj]);
     double e = I.etas[(size_t)jj * M.m + p_idx];
     I.E[(size_t)jj * cap + t] = e;
     ecol[jj] = e;
@@ -1125,7 +1147,7 @@ __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I
   if (threadIdx.x != 0) return;
   I.MF[(size_t)t * cap + t] = 1.0;
   I.eta_last[t] = lastt;
-  I.E[(size_t)t * cap + t] = eta_entry(I, u, p_idx);
+  I.E[(size_t)t * cap + t] = eta_value(u, p_idx, 0.0);
   I.eta_pos[t] = p_idx;
   S->t = t + 1;
   I.beta[p_idx] = u.bp / (u.ap * u.ap);
@@ -1180,9 +1202,72 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
   td = block_sumi<BS>(td);
   if (threadIdx.x == 0) {
     S->ctr[CTR_DUPD] = 0;
-    dupdate_final(M, I, phase, it, theta, td);
+    dupdate_final(M, I, phase, it, theta, td, I.alpha_q[S->p_idx]);
   }
   if (stable) update_final(M, I, phase, u);
+}
+
+// k_dupdate_update with the entering column's etas applied in the same launch
+// (eta count t > 0): every workgroup forms the eta coefficients from MF and
+// x0 = B0^{-1} a_q, takes alpha_q[p] from them, and writes alpha_q for its
+// rows as it updates them.  The last workgroup takes alpha_q at the older
+// etas' pivots from the E table (E[k][j] holds eta k at eta j's pivot), in the
+// same order of operations, so that it reads no entry another workgroup of
+// this launch wrote.  Results are bit-for-bit those of the separate FTRAN.
+__global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I, int phase, int it,
+                                                           const double* __restrict__ x0, int t) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  __shared__ double xm[ETA_MAX];
+  __shared__ double sk[ETA_MAX];
+  __shared__ int pk[ETA_MAX];
+  extern __shared__ double mft[];
+  eta_coeffs(I, x0, t, xm, sk, pk, mft);
+  const double theta = S->theta;
+  UpdateArgs u;
+  u.p_idx = S->p_idx;
+  u.ap = eta_apply(x0[u.p_idx], I.etas + u.p_idx, M.m, sk, pk, u.p_idx, t);
+  u.bp = I.beta[u.p_idx];
+  u.thP = (phase == 2 ? S->delta : S->f_p) / u.ap;
+  u.t = S->t;
+  u.eta = I.etas + (size_t)u.t * M.m;
+  const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
+  int dinf = 0;
+  dupdate_body(M, I, phase, theta, S->q_idx, dinf);
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    const double ai = eta_apply(x0[i], I.etas + i, M.m, sk, pk, i, t);
+    I.alpha_q[i] = ai;
+    if (!stable) continue;
+    if (phase == 2) {
+      int j = I.basis[i];
+      I.x[j] = I.x[j] - u.thP * ai;
+    } else if (i != u.p_idx) {
+      I.f[i] = I.f[i] - ai * u.thP;
+    }
+    if (i != u.p_idx) {
+      u.eta[i] = -ai / u.ap;
+      double r = ai / u.ap;
+      I.beta[i] = I.beta[i] - 2 * ai / u.ap * I.tau[i] + (r * r) * u.bp;
+    } else {
+      u.eta[i] = 1 / u.ap;
+    }
+  }
+  dinf = block_sumi<BS>(dinf);
+  if (threadIdx.x == 0) I.part_i[blockIdx.x] = dinf;
+  if (!last_block(&S->ctr[CTR_DUPD])) return;
+  int td = 0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) td += I.part_i[b];
+  td = block_sumi<BS>(td);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_DUPD] = 0;
+    dupdate_final(M, I, phase, it, theta, td, u.ap);
+  }
+  if (!stable) return;
+  // alpha_q at eta j's pivot, into xm (free after the coefficients).
+  const int cap = I.eta_cap;
+  for (int j = threadIdx.x; j < t; j += BS) xm[j] = eta_apply(x0[pk[j]], I.E + j, cap, sk, pk, pk[j], t);
+  __syncthreads();
+  update_final(M, I, phase, u, xm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1292,35 +1377,24 @@ __global__ __launch_bounds__(BS) void k_pratio1(DevModel M, DevIter I, int it) {
 // Pass 2: among blocking rows whose exact ratio is within theta_max, the
 // largest |alpha_q[i]| (ties to the lowest position) leaves; delta = x_p -
 // its bound, so the basis change moves x_q by delta / alpha_q[p].
-// BTRAN = true: the last workgroup goes on with BTRAN's set-up for the leaving
-// row, as k_price<true> does.
-template <bool BTRAN>
-__global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it, DevFactor F, int t) {
-  DevScalars* S = I.S;
-  if (stopped(S)) return;
-  const double dir = S->neg ? -1.0 : 1.0;
-  const double g = S->tmp0;
-  double best = -1.0;
-  int bi = INT_MAX;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+__device__ __forceinline__ void pratio2_scan(const DevModel& M, const DevIter& I, double dir, double g, int i0,
+                                             int stride, double& best, int& bi) {
+  for (int i = i0; i < M.m; i += stride) {
     double a, x, bd;
     if (!primal_row(M, I, dir, i, a, x, bd)) continue;
     if ((x - bd) / a <= g) argmax_merge(best, bi, fabs(a), i);
   }
-  block_argmax<BS>(best, bi);
-  if (threadIdx.x == 0) {
-    I.part_d[blockIdx.x] = best;
-    I.part_i[blockIdx.x] = bi;
-  }
-  if (!last_block(&S->ctr[CTR_CHUZC2])) return;
-  double fb = -1.0;
-  int fi = INT_MAX;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fb, fi, I.part_d[b], I.part_i[b]);
-  block_argmax<BS>(fb, fi);
+}
+// The leaving row fi (INT_MAX: none, ST_NO_PIVOT) is recorded; with BTRAN the
+// workgroup goes on with BTRAN's set-up for it, as k_price<true> does.  Every
+// thread of the workgroup calls it.
+template <bool BTRAN>
+__device__ __forceinline__ void pratio_leave(const DevModel& M, const DevIter& I, int it, const DevFactor& F, int t,
+                                             double dir, int fi) {
+  DevScalars* S = I.S;
   __shared__ int s_row;  // the leaving row, or -1 when an event was raised
   if (threadIdx.x == 0) {
     s_row = -1;
-    S->ctr[CTR_CHUZC2] = 0;
     if (fi == INT_MAX) {
       raise_status(S, ST_NO_PIVOT, it);
     } else {
@@ -1337,6 +1411,61 @@ __global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it, D
     __syncthreads();
     if (s_row >= 0) btran_prep_body(M, I, F, t, s_row, -1);
   }
+}
+template <bool BTRAN>
+__global__ __launch_bounds__(BS) void k_pratio2(DevModel M, DevIter I, int it, DevFactor F, int t) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double dir = S->neg ? -1.0 : 1.0;
+  double best = -1.0;
+  int bi = INT_MAX;
+  pratio2_scan(M, I, dir, S->tmp0, blockIdx.x * BS + threadIdx.x, gridDim.x * BS, best, bi);
+  block_argmax<BS>(best, bi);
+  if (threadIdx.x == 0) {
+    I.part_d[blockIdx.x] = best;
+    I.part_i[blockIdx.x] = bi;
+  }
+  if (!last_block(&S->ctr[CTR_CHUZC2])) return;
+  double fb = -1.0;
+  int fi = INT_MAX;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) argmax_merge(fb, fi, I.part_d[b], I.part_i[b]);
+  block_argmax<BS>(fb, fi);
+  if (threadIdx.x == 0) S->ctr[CTR_CHUZC2] = 0;
+  pratio_leave<BTRAN>(M, I, it, F, t, dir, fi);
+}
+
+// Both passes and BTRAN's set-up in one launch, for m up to PRATIO_ONE_WG:
+// the last workgroup of pass 1 scans every row for pass 2 itself (the same
+// minimum and the same arg-max, whatever the split of the rows).
+constexpr int PRATIO_ONE_WG = 8192;
+__global__ __launch_bounds__(BS) void k_pratio_btran(DevModel M, DevIter I, int it, DevFactor F, int t) {
+  DevScalars* S = I.S;
+  if (stopped(S)) return;
+  const double dir = S->neg ? -1.0 : 1.0;
+  double mn = BIG;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
+    double a, x, bd;
+    if (!primal_row(M, I, dir, i, a, x, bd)) continue;
+    const double r = a > 0 ? (x - bd + PRIMAL_TOL) / a : (x - bd - PRIMAL_TOL) / a;
+    mn = fmin(mn, fmax(r, 0.0));
+  }
+  mn = block_min<BS>(mn);
+  if (threadIdx.x == 0) I.part_d[blockIdx.x] = mn;
+  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  double g = BIG;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) g = fmin(g, I.part_d[b]);
+  g = block_min<BS>(g);
+  if (threadIdx.x == 0) {
+    S->ctr[CTR_CHUZC1] = 0;
+    S->tmp0 = g;
+    if (!(g < BIG)) raise_status(S, ST_NO_PIVOT, it);
+  }
+  if (!(g < BIG)) return;
+  double best = -1.0;
+  int bi = INT_MAX;
+  pratio2_scan(M, I, dir, g, threadIdx.x, BS, best, bi);
+  block_argmax<BS>(best, bi);
+  pratio_leave<true>(M, I, it, F, t, dir, bi);
 }
 
 // d_N -= theta alpha_r with the Devex weights w_j = max(w_j, (alpha_j /
@@ -1377,7 +1506,7 @@ __global__ __launch_bounds__(BS) void k_pupdate(DevModel M, DevIter I, int it) {
     S->ctr[CTR_DUPD] = 0;
     S->alpha_rq = arq;
     S->theta = theta;
-    dupdate_final(M, I, 2, it, theta, td);
+    dupdate_final(M, I, 2, it, theta, td, I.alpha_q[S->p_idx]);
     I.devex[S->p] = fmax(wq / (arq * arq), 1.0);
   }
   if (stable) update_final(M, I, 2, u);
@@ -2078,14 +2207,31 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   k_dupdate_update<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
 }
+// Above this many update rows the workgroups' redundant eta coefficients
+// (each reads MF's triangle) cost more than the launch they save.
+constexpr int UPDATE_ETA_MAX_N = 16384;
+void launch_dupdate_update_eta(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it,
+                               const double* x0, int t) {
+  const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
+  if (t > 0 && n > UPDATE_ETA_MAX_N) launch_ftran_eta(s, M, I, x0, I.alpha_q, t);
+  if (t == 0 || n > UPDATE_ETA_MAX_N) {
+    launch_dupdate_update(s, M, I, phase, it);
+    return;
+  }
+  k_dupdate_update_eta<<<cap_grid(n, BS, I.part_cap), BS, mf_lds_bytes(t), s>>>(M, I, phase, it, x0, t);
+}
 void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_pprice<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
 }
 void launch_pratio_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int it, int t,
                          double* rho) {
   const int g = cap_grid(M.m, BS, I.part_cap);
-  k_pratio1<<<g, BS, 0, s>>>(M, I, it);
-  k_pratio2<true><<<g, BS, mf_lds_bytes(t), s>>>(M, I, it, F, t);
+  if (M.m <= PRATIO_ONE_WG) {
+    k_pratio_btran<<<g, BS, mf_lds_bytes(t), s>>>(M, I, it, F, t);
+  } else {
+    k_pratio1<<<g, BS, 0, s>>>(M, I, it);
+    k_pratio2<true><<<g, BS, mf_lds_bytes(t), s>>>(M, I, it, F, t);
+  }
   k_btran_out<<<nblk(M.m, BO_ROWS), BS, 0, s>>>(M, I, F, rho, 1, -1);
 }
 void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
