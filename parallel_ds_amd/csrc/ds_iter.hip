@@ -874,11 +874,11 @@ __device__ __forceinline__ void eta_coeffs(const DevIter& I, const double* __res
 
 // Row i of the FTRAN result: x0[i] with the etas applied in order (an eta on
 // row i replaces the value, the others add to it).  The entries of eta k at
-// row i are read from stride-m storage (`e`, eta k at e[k * ld]).
+// row i are read from stride-m storage (`e`, eta k at e[k * ld]), EA of them
+// loaded ahead of their (order-dependent) application.
+template <int EA = 32>
 __device__ __forceinline__ double eta_apply(double x, const double* __restrict__ e, size_t ld, const double* sk,
                                             const int* pk, int i, int t) {
-  // Sixteen eta entries are loaded ahead of their (order-dependent) application.
-  constexpr int EA = 16;
   int k = 0;
   for (; k + EA <= t; k += EA) {
     double v[EA];
@@ -1119,9 +1119,10 @@ __device__ __forceinline__ double eta_entry(const DevIter& I, const UpdateArgs& 
 // pivots) and column t (older etas at the new pivot); then row t of
 // MF = (I - L)^{-1}, where L[t][i] = E[i][t] for i at or after the previous
 // eta on the same position; then (thread 0) the eta bookkeeping and the swap.
-// aq_pos (optional): alpha_q at the older etas' pivots, aq_pos[j] at eta j's.
+// aq_pos (optional): alpha_q at the older etas' pivots, aq_pos[j] at eta j's;
+// mft (optional): MF's lower triangle in LDS, rows packed (row i at tri(i)).
 __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I, int phase, const UpdateArgs& u,
-                                             const double* aq_pos = nullptr) {
+                                             const double* aq_pos = nullptr, const double* mft = nullptr) {
   DevScalars* S = I.S;
   const int t = u.t, p_idx = u.p_idx;
   const int cap = I.eta_cap;
@@ -1141,7 +1142,10 @@ __device__ __forceinline__ void update_final(const DevModel& M, const DevIter& I
   const int lastt = s_last;
   for (int j = threadIdx.x; j < t; j += BS) {
     double acc = 0.0;
-    for (int i = (j > lastt ? j : lastt); i < t; ++i) acc += ecol[i] * I.MF[(size_t)i * cap + j];
+    if (mft)
+      for (int i = (j > lastt ? j : lastt); i < t; ++i) acc += ecol[i] * mft[tri(i) + j];
+    else
+      for (int i = (j > lastt ? j : lastt); i < t; ++i) acc += ecol[i] * I.MF[(size_t)i * cap + j];
     I.MF[(size_t)t * cap + j] = acc;
   }
   if (threadIdx.x != 0) return;
@@ -1267,7 +1271,7 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I
   const int cap = I.eta_cap;
   for (int j = threadIdx.x; j < t; j += BS) xm[j] = eta_apply(x0[pk[j]], I.E + j, cap, sk, pk, pk[j], t);
   __syncthreads();
-  update_final(M, I, phase, u, xm);
+  update_final(M, I, phase, u, xm, t <= MF_TRI_MAX ? mft : nullptr);
 }
 
 // ---------------------------------------------------------------------------
