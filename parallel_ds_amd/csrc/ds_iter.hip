@@ -756,6 +756,76 @@ __global__ __launch_bounds__(BS) void k_chuzc2(DevModel M, DevIter I, int phase,
 // select rows of K^{-T}; slack positions subtract the coupling block.
 // ---------------------------------------------------------------------------
 constexpr int FT_MAXCOL = 512;
+// Row i's factor data, which does not depend on the column: its kernel column
+// c (or -1 for a slack-covered row), that row r and its coupling range.
+struct ColRow {
+  int c, r, qb, qe;
+};
+__device__ __forceinline__ ColRow col_row(const DevFactor& F, int i) {
+  ColRow w;
+  w.c = F.Pidx_of_pos[i];
+  w.r = w.c < 0 ? F.slackrow_of_pos[i] : 0;
+  w.qb = w.c < 0 ? F.crp[w.r] : 0;
+  w.qe = w.c < 0 ? F.crp[w.r + 1] : 0;
+  return w;
+}
+// Position i of x0 = B0^{-1} a for the column held in er/eri/ev (ne entries).
+__device__ __forceinline__ double ftran_col_value(const DevFactor& F, const ColRow& w, int ne, const int* er,
+                                                  const int* eri, const double* ev) {
+  const int k = F.k;
+  double x;
+  if (w.c >= 0) {
+    x = 0.0;
+    for (int e = 0; e < ne; ++e)
+      if (eri[e] >= 0) x += ev[e] * F.KinvT[(size_t)eri[e] * k + w.c];
+  } else {
+    double ar = 0.0;
+    for (int e = 0; e < ne; ++e)
+      if (er[e] == w.r) ar = ev[e];
+    // Coupling entries four at a time, so their loads overlap; each x_P value
+    // and the sum over q keep the order of a one-at-a-time loop.
+    constexpr int U = 4;
+    double s = 0.0;
+    for (int q0 = w.qb; q0 < w.qe; q0 += U) {
+      int c2[U];
+      double cv[U], xp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = q0 + u < w.qe;
+        c2[u] = ok ? F.cci[q0 + u] : 0;
+        cv[u] = ok ? F.ccv[q0 + u] : 0.0;
+        xp[u] = 0.0;
+      }
+      for (int e = 0; e < ne; ++e)
+        if (eri[e] >= 0) {
+          const double* row = F.KinvT + (size_t)eri[e] * k;
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (q0 + u < w.qe) xp[u] += ev[e] * row[c2[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + u < w.qe) s += cv[u] * xp[u];
+    }
+    x = ar - s;
+  }
+  return x;
+}
+// The column's entries into LDS (row, its kernel row or -1, value); returns
+// their count.  No barrier.
+__device__ __forceinline__ int load_col(const DevModel& M, const DevFactor& F, int j, int* er, int* eri,
+                                        double* ev) {
+  const int cb = M.cptr[j];
+  const int ne = min(M.cptr[j + 1] - cb, FT_MAXCOL);
+  for (int e = threadIdx.x; e < ne; e += BS) {
+    int rr = M.ridx[cb + e];
+    er[e] = rr;
+    eri[e] = F.Ridx_of_row[rr];
+    ev[e] = M.cval[cb + e];
+  }
+  return ne;
+}
+
 __device__ __forceinline__ void ftran_col_body(const DevModel& M, const DevIter& I, const DevFactor& F, int col,
                                                double* out, int check, int bid) {
   DevScalars* S = I.S;
@@ -767,58 +837,11 @@ __device__ __forceinline__ void ftran_col_body(const DevModel& M, const DevIter&
   // The row's own factor data does not depend on the column: fetch it first.
   const int i = bid * BS + threadIdx.x;
   const bool live = i < M.m;
-  const int c = live ? F.Pidx_of_pos[i] : 0;
-  const int r = (live && c < 0) ? F.slackrow_of_pos[i] : 0;
-  const int qb = (live && c < 0) ? F.crp[r] : 0;
-  const int qe = (live && c < 0) ? F.crp[r + 1] : 0;
-  const int cb = M.cptr[j];
-  const int ne = min(M.cptr[j + 1] - cb, FT_MAXCOL);
-  for (int e = threadIdx.x; e < ne; e += BS) {
-    int rr = M.ridx[cb + e];
-    er[e] = rr;
-    eri[e] = F.Ridx_of_row[rr];
-    ev[e] = M.cval[cb + e];
-  }
+  const ColRow w = live ? col_row(F, i) : ColRow{0, 0, 0, 0};
+  const int ne = load_col(M, F, j, er, eri, ev);
   __syncthreads();
   if (!live) return;
-  const int k = F.k;
-  double x;
-  if (c >= 0) {
-    x = 0.0;
-    for (int e = 0; e < ne; ++e)
-      if (eri[e] >= 0) x += ev[e] * F.KinvT[(size_t)eri[e] * k + c];
-  } else {
-    double ar = 0.0;
-    for (int e = 0; e < ne; ++e)
-      if (er[e] == r) ar = ev[e];
-    // Coupling entries four at a time, so their loads overlap; each x_P value
-    // and the sum over q keep the order of a one-at-a-time loop.
-    constexpr int U = 4;
-    double s = 0.0;
-    for (int q0 = qb; q0 < qe; q0 += U) {
-      int c2[U];
-      double cv[U], xp[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const bool ok = q0 + u < qe;
-        c2[u] = ok ? F.cci[q0 + u] : 0;
-        cv[u] = ok ? F.ccv[q0 + u] : 0.0;
-        xp[u] = 0.0;
-      }
-      for (int e = 0; e < ne; ++e)
-        if (eri[e] >= 0) {
-          const double* row = F.KinvT + (size_t)eri[e] * k;
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-            if (q0 + u < qe) xp[u] += ev[e] * row[c2[u]];
-        }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (q0 + u < qe) s += cv[u] * xp[u];
-    }
-    x = ar - s;
-  }
-  out[i] = x;
+  out[i] = ftran_col_value(F, w, ne, er, eri, ev);
 }
 
 __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFactor F, int col, double* out,
@@ -830,17 +853,22 @@ __global__ __launch_bounds__(BS) void k_ftran_col(DevModel M, DevIter I, DevFact
 // xm[k] = x0 at eta k's pivot when no earlier eta has that pivot (else 0),
 // sk = MF xm, pk[k] = eta k's pivot.  Every thread of the workgroup takes part;
 // ends with a barrier.
+__device__ __forceinline__ void eta_solve(const DevIter& I, int t, const double* xm, double* sk, const double* mft);
 __device__ __forceinline__ void eta_coeffs(const DevIter& I, const double* __restrict__ x0, int t, double* xm,
                                            double* sk, int* pk, double* mft) {
-  const int cap = I.eta_cap;
-  const bool tile = t <= MF_TRI_MAX;
   for (int k = threadIdx.x; k < t; k += BS) {
     int q = I.eta_pos[k];
     pk[k] = q;
     xm[k] = (I.eta_last[k] < 0) ? x0[q] : 0.0;
   }
-  if (tile) load_mf_tri(I, t, mft);
+  if (t <= MF_TRI_MAX) load_mf_tri(I, t, mft);
   __syncthreads();
+  eta_solve(I, t, xm, sk, mft);
+}
+// sk = MF xm (MF's triangle in LDS when t <= MF_TRI_MAX); ends with a barrier.
+__device__ __forceinline__ void eta_solve(const DevIter& I, int t, const double* xm, double* sk, const double* mft) {
+  const int cap = I.eta_cap;
+  const bool tile = t <= MF_TRI_MAX;
   // s = MF xm: one thread per row of MF with four independent partial sums;
   // the row comes from LDS, or from global memory in 16-byte pairs.
   for (int k = threadIdx.x; k < t; k += BS) {
@@ -923,6 +951,41 @@ __device__ __forceinline__ void ftran_out_body(const DevModel& M, const DevIter&
 __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const double* __restrict__ x0,
                                                   double* out, int t, int mode, int check) {
   ftran_out_body(M, I, x0, out, t, mode, check, blockIdx.x);
+}
+
+// FTRAN of a column with the etas in one launch (t > 0): every workgroup
+// computes x0 = B0^{-1} a at the etas' pivots itself (the same arithmetic as
+// k_ftran_col), forms the eta coefficients, then writes x0 and the result for
+// its own rows.  Results are bit-for-bit those of k_ftran_col + k_ftran_out.
+__global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, DevFactor F, int col, double* out,
+                                                      int t, int check) {
+  DevScalars* S = I.S;
+  if (skip_launch(S, check)) return;
+  const int j = col >= 0 ? col : S->q;
+  __shared__ int er[FT_MAXCOL];
+  __shared__ int eri[FT_MAXCOL];
+  __shared__ double ev[FT_MAXCOL];
+  __shared__ double xm[ETA_MAX];
+  __shared__ double sk[ETA_MAX];
+  __shared__ int pk[ETA_MAX];
+  extern __shared__ double mft[];
+  const int i = blockIdx.x * BS + threadIdx.x;
+  const bool live = i < M.m;
+  const ColRow w = live ? col_row(F, i) : ColRow{0, 0, 0, 0};
+  const int ne = load_col(M, F, j, er, eri, ev);
+  if (t <= MF_TRI_MAX) load_mf_tri(I, t, mft);
+  __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS) {
+    const int q = I.eta_pos[k];
+    pk[k] = q;
+    xm[k] = (I.eta_last[k] < 0) ? ftran_col_value(F, col_row(F, q), ne, er, eri, ev) : 0.0;
+  }
+  __syncthreads();
+  eta_solve(I, t, xm, sk, mft);
+  if (!live) return;
+  const double x0 = ftran_col_value(F, w, ne, er, eri, ev);
+  I.x0[i] = x0;
+  out[i] = eta_apply(x0, I.etas + i, M.m, sk, pk, i, t);
 }
 
 // Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
@@ -2175,8 +2238,7 @@ void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase,
 void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
                       double* out, int check) {
   if (t > 0) {
-    k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, I.x0, check);
-    k_ftran_out<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, I.x0, out, t, 0, check);
+    k_ftran_col_eta<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, F, col, out, t, check);
   } else {
     k_ftran_col<<<nblk(M.m, BS), BS, 0, s>>>(M, I, F, col, out, check);
   }

@@ -128,7 +128,8 @@ def test_transport_vs_oracle(gpu, S, D, seed):
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_btran_ftran_large_kernel(gpu, seed):
-    """A basis whose kernel exceeds one tournament chunk (k > 256): multi-round pivoting."""
+    """A basis whose kernel exceeds one tournament chunk (k = 341 > 256): the panels take the
+    single-round tournament of eight waves (up to 512 rows per panel)."""
     L = lpmod.gen_bernoulli(600, 1200, 0.02, 40 + seed)
     r_o, x_o, b_o = ora.solve(L)
     assert r_o.status == 0
@@ -182,19 +183,22 @@ def test_time_pivot_row_ext(gpu):
     assert by == 12.0 * L.nnz + 4.0 * (L.n + 1) + 8.0 * L.n + 8.0 * L.m
 
 
-@pytest.mark.parametrize("dependent", [False, True])
-def test_refactor_flags_a_numerically_singular_kernel(gpu, dependent):
-    """A 300 x 300 basis kernel (two tournament rounds in the first panels) with one column an
-    exact combination of two others. Rounding leaves its pivot near 1e-16 of the row's scale, not
-    at zero; the refactorization must still report it singular (DS_ESINGULAR). Without the
-    dependency the same basis factors and FTRAN matches a dense solve."""
+@pytest.mark.parametrize("k,dep", [(300, None), (300, (2, 0, 1)), (900, None), (900, (200, 150, 190))])
+def test_refactor_flags_a_numerically_singular_kernel(gpu, k, dep):
+    """A k x k basis kernel (k = 300: one tournament round per panel; k = 900: panels with more
+    than 512 rows left merge candidate lists over several rounds) with, in the dependent cases,
+    one column an exact combination of two earlier ones (k = 900: column 200, in panel 3 of the
+    merge path).  Rounding leaves its pivot near 1e-16 of the row's scale, not at zero; the
+    refactorization must still report it singular (DS_ESINGULAR).  Without the dependency the
+    same basis factors, and FTRAN and BTRAN match dense solves."""
     from parallel_ds_amd._native import DsError
     rng = np.random.default_rng(5)
-    m, n, k = 320, 400, 300
+    m, n = k + 20, k + 100
     A = (rng.random((m, n)) < 0.05) * rng.uniform(0.1, 1.0, (m, n))
     A[np.arange(m), np.arange(m) % n] += rng.uniform(0.5, 1.0, m)
-    if dependent:
-        A[:, 2] = A[:, 0] + 0.7 * A[:, 1]
+    if dep is not None:
+        c, c0, c1 = dep
+        A[:, c] = A[:, c0] + 0.7 * A[:, c1]
     import scipy.sparse as sp
     C = sp.csc_matrix(A)
     C.sort_indices()
@@ -203,7 +207,7 @@ def test_refactor_flags_a_numerically_singular_kernel(gpu, dependent):
     basis = np.concatenate([np.arange(k), n + np.arange(k, m)]).astype(np.int32)
     with DualSimplex(L, CAP) as s:
         s.initDualSimplex()
-        if dependent:
+        if dep is not None:
             with pytest.raises(DsError) as e:
                 s.set_basis(basis)
             assert e.value.code == -3
@@ -214,3 +218,9 @@ def test_refactor_flags_a_numerically_singular_kernel(gpu, dependent):
             x = s.ftran(b)
             ref = np.linalg.solve(B, b)
             assert np.abs(x - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+            for p in (0, k // 2, m - 1):
+                rho = s.btran(p)
+                e = np.zeros(m)
+                e[p] = 1
+                ref = np.linalg.solve(B.T, e)
+                assert np.abs(rho - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
