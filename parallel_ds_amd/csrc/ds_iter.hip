@@ -1274,26 +1274,48 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
   if (stable) update_final(M, I, phase, u);
 }
 
-// k_dupdate_update with the entering column's etas applied in the same launch
-// (eta count t > 0): every workgroup forms the eta coefficients from MF and
-// x0 = B0^{-1} a_q, takes alpha_q[p] from them, and writes alpha_q for its
-// rows as it updates them.  The last workgroup takes alpha_q at the older
-// etas' pivots from the E table (E[k][j] holds eta k at eta j's pivot), in the
-// same order of operations, so that it reads no entry another workgroup of
-// this launch wrote.  Results are bit-for-bit those of the separate FTRAN.
-__global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I, int phase, int it,
-                                                           const double* __restrict__ x0, int t) {
+// The second half of a dual iteration in one launch: the entering column's
+// FTRAN, its etas (alpha_q) and the DSE vector's etas (tau0 -> tau), then the
+// dual update and the basis change of k_dupdate_update.  Every workgroup
+// computes x0 = B0^{-1} a_q at the eta pivots (the arithmetic of k_ftran_col)
+// and reads tau0 there, forms both sets of eta coefficients from MF's
+// triangle (in LDS), and writes x0, alpha_q and tau for its rows as it
+// updates them.  The last workgroup takes alpha_q at the older etas' pivots
+// from the E table (E[k][j] holds eta k at eta j's pivot) in the same order
+// of operations, so that it reads no entry another workgroup of this launch
+// wrote.  Results are bit-for-bit those of k_stage_col_out, k_ftran_out and
+// k_dupdate_update.  tau0 is tau itself when t == 0.
+__global__ __launch_bounds__(BS) void k_dupdate_update_col(DevModel M, DevIter I, DevFactor F, int phase, int it,
+                                                           const double* tau0, int t) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
-  __shared__ double xm[ETA_MAX];
+  __shared__ int er[FT_MAXCOL];
+  __shared__ int eri[FT_MAXCOL];
+  __shared__ double ev[FT_MAXCOL];
+  __shared__ double xm[ETA_MAX];  // x0 at the pivots, then (last workgroup) alpha_q there
   __shared__ double sk[ETA_MAX];
+  __shared__ double xt[ETA_MAX];  // tau0 at the pivots
+  __shared__ double st[ETA_MAX];
   __shared__ int pk[ETA_MAX];
   extern __shared__ double mft[];
-  eta_coeffs(I, x0, t, xm, sk, pk, mft);
+  const int ne = load_col(M, F, S->q, er, eri, ev);
+  if (t <= MF_TRI_MAX) load_mf_tri(I, t, mft);
+  __syncthreads();
+  for (int k = threadIdx.x; k < t; k += BS) {
+    const int q = I.eta_pos[k];
+    const bool first = I.eta_last[k] < 0;
+    pk[k] = q;
+    xm[k] = first ? ftran_col_value(F, col_row(F, q), ne, er, eri, ev) : 0.0;
+    xt[k] = first ? tau0[q] : 0.0;
+  }
+  __syncthreads();
+  eta_solve(I, t, xm, sk, mft);
+  eta_solve(I, t, xt, st, mft);
   const double theta = S->theta;
   UpdateArgs u;
   u.p_idx = S->p_idx;
-  u.ap = eta_apply(x0[u.p_idx], I.etas + u.p_idx, M.m, sk, pk, u.p_idx, t);
+  u.ap = eta_apply(ftran_col_value(F, col_row(F, u.p_idx), ne, er, eri, ev), I.etas + u.p_idx, M.m, sk, pk,
+                   u.p_idx, t);
   u.bp = I.beta[u.p_idx];
   u.thP = (phase == 2 ? S->delta : S->f_p) / u.ap;
   u.t = S->t;
@@ -1302,8 +1324,12 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I
   int dinf = 0;
   dupdate_body(M, I, phase, theta, S->q_idx, dinf);
   for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
-    const double ai = eta_apply(x0[i], I.etas + i, M.m, sk, pk, i, t);
+    const double x0 = ftran_col_value(F, col_row(F, i), ne, er, eri, ev);
+    const double ai = eta_apply(x0, I.etas + i, M.m, sk, pk, i, t);
+    const double ti = eta_apply(tau0[i], I.etas + i, M.m, st, pk, i, t);
+    I.x0[i] = x0;
     I.alpha_q[i] = ai;
+    I.tau[i] = ti;
     if (!stable) continue;
     if (phase == 2) {
       int j = I.basis[i];
@@ -1314,7 +1340,7 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I
     if (i != u.p_idx) {
       u.eta[i] = -ai / u.ap;
       double r = ai / u.ap;
-      I.beta[i] = I.beta[i] - 2 * ai / u.ap * I.tau[i] + (r * r) * u.bp;
+      I.beta[i] = I.beta[i] - 2 * ai / u.ap * ti + (r * r) * u.bp;
     } else {
       u.eta[i] = 1 / u.ap;
     }
@@ -1332,7 +1358,10 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_eta(DevModel M, DevIter I
   if (!stable) return;
   // alpha_q at eta j's pivot, into xm (free after the coefficients).
   const int cap = I.eta_cap;
-  for (int j = threadIdx.x; j < t; j += BS) xm[j] = eta_apply(x0[pk[j]], I.E + j, cap, sk, pk, pk[j], t);
+  for (int j = threadIdx.x; j < t; j += BS) {
+    const double x0 = ftran_col_value(F, col_row(F, pk[j]), ne, er, eri, ev);
+    xm[j] = eta_apply(x0, I.E + j, cap, sk, pk, pk[j], t);
+  }
   __syncthreads();
   update_final(M, I, phase, u, xm, t <= MF_TRI_MAX ? mft : nullptr);
 }
@@ -2274,17 +2303,18 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
   k_dupdate_update<<<cap_grid(n, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
 }
 // Above this many update rows the workgroups' redundant eta coefficients
-// (each reads MF's triangle) cost more than the launch they save.
+// (each reads MF's triangle) cost more than the launches they save.
 constexpr int UPDATE_ETA_MAX_N = 16384;
-void launch_dupdate_update_eta(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it,
-                               const double* x0, int t) {
+void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
+                       int t, const double* tau0) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
-  if (t > 0 && n > UPDATE_ETA_MAX_N) launch_ftran_eta(s, M, I, x0, I.alpha_q, t);
-  if (t == 0 || n > UPDATE_ETA_MAX_N) {
+  if (n > UPDATE_ETA_MAX_N) {
+    launch_stage_col_out(s, M, I, F, t, t > 0 ? I.x0 : I.alpha_q, tau0, I.tau);
+    launch_ftran_eta(s, M, I, I.x0, I.alpha_q, t);
     launch_dupdate_update(s, M, I, phase, it);
     return;
   }
-  k_dupdate_update_eta<<<cap_grid(n, BS, I.part_cap), BS, mf_lds_bytes(t), s>>>(M, I, phase, it, x0, t);
+  k_dupdate_update_col<<<cap_grid(n, BS, I.part_cap), BS, mf_lds_bytes(t), s>>>(M, I, F, phase, it, tau0, t);
 }
 void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_pprice<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);

@@ -23,8 +23,8 @@ void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase,
 // bound flips): pivot-row SpMV + tau's kernel product into tau0; CHUZC +
 // tau's slack rows; the entering column's FTRAN (into colout: x0 when t > 0,
 // else alpha_q) + tau's etas (tau0 -> tau); then the column's etas
-// (launch_ftran_eta, x0 -> alpha_q, or inside launch_dupdate_update_eta).
-// tau0 is tau itself when t == 0.
+// (launch_ftran_eta, x0 -> alpha_q); or all of the last two inside
+// launch_col_update.  tau0 is tau itself when t == 0.
 void launch_stage_spmv_gemv(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const int* posN,
                             const double* rho, const double* cvec, double* alpha_r, double* tau0);
 void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase,
@@ -41,10 +41,12 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
 void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
-// The same with the entering column's etas applied in the launch: x0 is the
-// column's eta-free FTRAN, alpha_q is written as the rows are updated.
-void launch_dupdate_update_eta(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it,
-                               const double* x0, int t);
+// The rest of a fused dual iteration after CHUZC: the entering column's FTRAN
+// with its etas, tau's etas (tau0 -> tau), the dual update and the basis
+// change; one launch up to 16,384 update rows, else launch_stage_col_out +
+// launch_ftran_eta + launch_dupdate_update.
+void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
+                       int t, const double* tau0);
 // Primal simplex iterations of a warm start: entering column (Devex), the
 // two-pass ratio test over the basic rows, and the update (after FTRAN, BTRAN
 // and the pivot-row SpMV).

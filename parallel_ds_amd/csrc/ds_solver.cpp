@@ -740,7 +740,8 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
     // CHUZR goes on with BTRAN's set-up, and the DSE FTRAN's stages share the
     // launches of the SpMV, CHUZC and the entering column's FTRAN (same
     // kernels' code, same results), and up to 16,384 update rows the entering
-    // column's etas are applied by the update: six launches per iteration.
+    // column's FTRAN and both sets of etas run in the update's launch: five
+    // launches per iteration.
     launch_price_btran(st_, m, i, f, phase, it, skip_top, P_.perturb, t, rho_.p);
     double* tau0 = t > 0 ? w2_.p : tau_.p;
     launch_stage_spmv_gemv(st_, m, i, f, posN_.p, rho_.p, c_.p, alpha_r_.p, tau0);
@@ -748,8 +749,7 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
     if (phase == 2 && force_due(DS_FORCE_STABILITY, it)) {
       launch_force(st_, m, i, DS_FORCE_STABILITY);
     }
-    launch_stage_col_out(st_, m, i, f, t, t > 0 ? x0_.p : alpha_q_.p, tau0, tau_.p);
-    launch_dupdate_update_eta(st_, m, i, phase, it, x0_.p, t);
+    launch_col_update(st_, m, i, f, phase, it, t, tau0);
     return;
   }
   launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
