@@ -342,22 +342,19 @@ __global__ __launch_bounds__(WV * WAVE) void k_tslu(const double* W, int k, int 
 // chosen row in W (before the panel's exchanges), matching the tournament's
 // treatment, so the trailing update leaves the other columns as they would be
 // for a basis with that row's slack in place of the column.
-__global__ __launch_bounds__(256) void k_dep_columns(double* W, int k, int j, int nb, const int* flag) {
-  const int n = min(flag[1], GJ_DEP_CAP);
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= k) return;
-  for (int d = 0; d < n; ++d) {
-    const int step = flag[2 + 2 * d];
-    if (step < j || step >= j + nb) continue;
-    W[(size_t)i * k + step] = (i == flag[3 + 2 * d]) ? 1.0 : 0.0;
-  }
-}
-
-// Apply the panel's row swaps to every column.
+// Thread per column c of W: a column that the tournament found dependent in
+// this panel becomes the unit column of the row it was given; then the
+// panel's row exchanges, in order (rowscale follows them in thread 0).
 __global__ __launch_bounds__(256) void k_swap_rows(double* W, int k, int j, int nb, const int* piv,
-                                                   double* rowscale) {
+                                                   double* rowscale, const int* flag) {
   int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= k) return;
+  const int n = min(flag[1], GJ_DEP_CAP);
+  for (int d = 0; d < n; ++d) {
+    if (flag[2 + 2 * d] != c || c < j || c >= j + nb) continue;
+    const int row = flag[3 + 2 * d];
+    for (int i = 0; i < k; ++i) W[(size_t)i * k + c] = (i == row) ? 1.0 : 0.0;
+  }
   for (int s = 0; s < nb; ++s) {
     int r = piv[j + s];
     if (r != j + s && r > j + s && r < k) {
@@ -372,7 +369,6 @@ __global__ __launch_bounds__(256) void k_swap_rows(double* W, int k, int j, int 
     }
   }
 }
-
 // rowscale[r] = max_c |W[r][c]| (one wave per row).
 __global__ __launch_bounds__(256) void k_row_scale(const double* W, int k, double* rowscale) {
   const int r = (blockIdx.x * 256 + threadIdx.x) / WAVE;
@@ -384,28 +380,33 @@ __global__ __launch_bounds__(256) void k_row_scale(const double* W, int k, doubl
   if (lane == 0) rowscale[r] = v;
 }
 
-// Save the (swapped) panel columns and reset them to identity columns.
-__global__ __launch_bounds__(256) void k_panel_save(double* W, int k, int j, int nb, double* Pb) {
-  long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long)k * nb) return;
-  int r = (int)(e / nb), cc = (int)(e % nb);
-  double* w = &W[(size_t)r * k + j + cc];
-  Pb[(size_t)r * NB + cc] = *w;
-  *w = (r == j + cc) ? 1.0 : 0.0;
-}
-
-// Mid rows: X <- Pinv X for every column, Pinv = inverse of the pivot block.
-__global__ __launch_bounds__(256) void k_mid_gemm(double* W, int k, int j, int nb, const double* Pinv) {
+// Mid rows: the pivot rows j..j+nb of W become Pinv times themselves, their panel
+// columns taken as the identity; and the panel (columns j..j+nb) of the other
+// rows moves to Pb, leaving zeros (the pivot rows of Pb are zero: the
+// trailing update skips them).  The two parts touch disjoint entries of W.
+__global__ __launch_bounds__(256) void k_mid_gemm(double* W, int k, int j, int nb, const double* Pinv, double* Pb) {
   __shared__ double Pi[NB][NB + 1];
   __shared__ double X[NB][64 + 1];
   const int c0 = blockIdx.x * 64;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)k * nb; e += (long)gridDim.x * 256) {
+    const int r = (int)(e / nb), cc = (int)(e % nb);
+    if (r >= j && r < j + nb) {
+      Pb[(size_t)r * NB + cc] = 0.0;
+    } else {
+      double* w = &W[(size_t)r * k + j + cc];
+      Pb[(size_t)r * NB + cc] = *w;
+      *w = 0.0;
+    }
+  }
   for (int e = threadIdx.x; e < NB * NB; e += 256) {
     int r = e / NB, t = e % NB;
     Pi[r][t] = (r < nb && t < nb) ? Pinv[(size_t)r * NB + t] : 0.0;
   }
   for (int e = threadIdx.x; e < NB * 64; e += 256) {
     int r = e / 64, cc = e % 64;
-    X[r][cc] = (r < nb && c0 + cc < k) ? W[(size_t)(j + r) * k + c0 + cc] : 0.0;
+    const int c = c0 + cc;
+    X[r][cc] = (r < nb && c < k) ? ((c >= j && c < j + nb) ? (c - j == r ? 1.0 : 0.0) : W[(size_t)(j + r) * k + c])
+                                 : 0.0;
   }
   __syncthreads();
   const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
@@ -564,10 +565,8 @@ void launch_gj_panels(hipStream_t s, int k, const GJWork& G) {
       out_n = tn;
       groups = g2;
     }
-    k_dep_columns<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.flag);
-    k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv, G.rowscale);
-    k_panel_save<<<(int)(((long)k * nb + 255) / 256), 256, 0, s>>>(G.W, k, j, nb, G.Pb);
-    k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv);
+    k_swap_rows<<<(k + 255) / 256, 256, 0, s>>>(G.W, k, j, nb, G.piv, G.rowscale, G.flag);
+    k_mid_gemm<<<(k + 63) / 64, 256, 0, s>>>(G.W, k, j, nb, G.Pinv, G.Pb);
     dim3 grid((k + 63) / 64, (k + 63) / 64);
     if (G.use_mfma && G.gemm_ev)
       hipExtLaunchKernelGGL(k_gemm_sub, grid, dim3(256), 0, s, G.gemm_ev[2 * (j / NB)], G.gemm_ev[2 * (j / NB) + 1], 0,
