@@ -1292,30 +1292,49 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_col(DevModel M, DevIter I
   __shared__ int er[FT_MAXCOL];
   __shared__ int eri[FT_MAXCOL];
   __shared__ double ev[FT_MAXCOL];
-  __shared__ double xm[ETA_MAX];  // x0 at the pivots, then (last workgroup) alpha_q there
+  __shared__ double xm[ETA_MAX];  // x0 at the first etas' pivots, then (last workgroup) alpha_q there
   __shared__ double sk[ETA_MAX];
   __shared__ double xt[ETA_MAX];  // tau0 at the pivots
   __shared__ double st[ETA_MAX];
   __shared__ int pk[ETA_MAX];
+  __shared__ double x0p[ETA_MAX + 1];  // x0 at every eta's pivot, [t] at the leaving row
+  __shared__ double ep[ETA_MAX];       // the etas' entries at the leaving row
   extern __shared__ double mft[];
+  const int p_idx = S->p_idx;
+  // The first row's factor data does not depend on the column: fetch it first.
+  const int i0 = blockIdx.x * BS + threadIdx.x;
+  const ColRow w0 = i0 < M.m ? col_row(F, i0) : ColRow{0, 0, 0, 0};
   const int ne = load_col(M, F, S->q, er, eri, ev);
   if (t <= MF_TRI_MAX) load_mf_tri(I, t, mft);
   __syncthreads();
-  for (int k = threadIdx.x; k < t; k += BS) {
-    const int q = I.eta_pos[k];
-    const bool first = I.eta_last[k] < 0;
-    pk[k] = q;
-    xm[k] = first ? ftran_col_value(F, col_row(F, q), ne, er, eri, ev) : 0.0;
-    xt[k] = first ? tau0[q] : 0.0;
+  for (int k = threadIdx.x; k <= t; k += BS) {
+    const int q = k < t ? I.eta_pos[k] : p_idx;
+    const double x = ftran_col_value(F, col_row(F, q), ne, er, eri, ev);
+    x0p[k] = x;
+    if (k < t) {
+      const bool first = I.eta_last[k] < 0;
+      pk[k] = q;
+      xm[k] = first ? x : 0.0;
+      xt[k] = first ? tau0[q] : 0.0;
+      ep[k] = I.etas[(size_t)k * M.m + p_idx];
+    }
   }
   __syncthreads();
   eta_solve(I, t, xm, sk, mft);
   eta_solve(I, t, xt, st, mft);
   const double theta = S->theta;
   UpdateArgs u;
-  u.p_idx = S->p_idx;
-  u.ap = eta_apply(ftran_col_value(F, col_row(F, u.p_idx), ne, er, eri, ev), I.etas + u.p_idx, M.m, sk, pk,
-                   u.p_idx, t);
+  u.p_idx = p_idx;
+  {
+    // alpha_q[p]: the etas applied in order, from the entries staged in LDS.
+    double x = x0p[t];
+    for (int k = 0; k < t; ++k) {
+      const double sv = sk[k];
+      if (pk[k] == p_idx) x = sv * ep[k];
+      else if (sv != 0.0) x = x + sv * ep[k];
+    }
+    u.ap = x;
+  }
   u.bp = I.beta[u.p_idx];
   u.thP = (phase == 2 ? S->delta : S->f_p) / u.ap;
   u.t = S->t;
@@ -1323,10 +1342,10 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_col(DevModel M, DevIter I
   const bool stable = !(fabs(u.ap - S->alpha_rq) > EPS_R * (1 + fabs(u.ap)));
   int dinf = 0;
   dupdate_body(M, I, phase, theta, S->q_idx, dinf);
-  for (int i = blockIdx.x * BS + threadIdx.x; i < M.m; i += gridDim.x * BS) {
-    const double x0 = ftran_col_value(F, col_row(F, i), ne, er, eri, ev);
-    const double ai = eta_apply(x0, I.etas + i, M.m, sk, pk, i, t);
-    const double ti = eta_apply(tau0[i], I.etas + i, M.m, st, pk, i, t);
+  for (int i = i0; i < M.m; i += gridDim.x * BS) {
+    const double x0 = ftran_col_value(F, i == i0 ? w0 : col_row(F, i), ne, er, eri, ev);
+    const double ai = eta_apply<64>(x0, I.etas + i, M.m, sk, pk, i, t);
+    const double ti = eta_apply<64>(tau0[i], I.etas + i, M.m, st, pk, i, t);
     I.x0[i] = x0;
     I.alpha_q[i] = ai;
     I.tau[i] = ti;
@@ -1358,10 +1377,7 @@ __global__ __launch_bounds__(BS) void k_dupdate_update_col(DevModel M, DevIter I
   if (!stable) return;
   // alpha_q at eta j's pivot, into xm (free after the coefficients).
   const int cap = I.eta_cap;
-  for (int j = threadIdx.x; j < t; j += BS) {
-    const double x0 = ftran_col_value(F, col_row(F, pk[j]), ne, er, eri, ev);
-    xm[j] = eta_apply(x0, I.E + j, cap, sk, pk, pk[j], t);
-  }
+  for (int j = threadIdx.x; j < t; j += BS) xm[j] = eta_apply<64>(x0p[j], I.E + j, cap, sk, pk, pk[j], t);
   __syncthreads();
   update_final(M, I, phase, u, xm, t <= MF_TRI_MAX ? mft : nullptr);
 }
