@@ -985,7 +985,7 @@ __global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, Dev
   if (!live) return;
   const double x0 = ftran_col_value(F, w, ne, er, eri, ev);
   I.x0[i] = x0;
-  out[i] = eta_apply(x0, I.etas + i, M.m, sk, pk, i, t);
+  out[i] = eta_apply<64>(x0, I.etas + i, M.m, sk, pk, i, t);
 }
 
 // Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
@@ -1497,6 +1497,42 @@ __device__ __forceinline__ void pratio2_scan(const DevModel& M, const DevIter& I
     if ((x - bd) / a <= g) argmax_merge(best, bi, fabs(a), i);
   }
 }
+// The same scan by one workgroup over every row, U rows per thread at a time
+// with their loads issued together (the rows are independent; the arg-max does
+// not depend on the order).
+__device__ __forceinline__ void pratio2_scan_wg(const DevModel& M, const DevIter& I, double dir, double g,
+                                                double& best, int& bi) {
+  constexpr int U = 8;
+  for (int i0 = threadIdx.x; i0 < M.m; i0 += BS * U) {
+    double a[U], x[U], lb[U], ub[U];
+    int j[U];
+    int8_t bt[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * BS;
+      a[u] = i < M.m ? dir * I.alpha_q[i] : 0.0;
+      j[u] = i < M.m ? I.basis[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bt[u] = M.bt[j[u]];
+      x[u] = I.x[j[u]];
+      lb[u] = M.l[j[u]];
+      ub[u] = M.u[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * BS;
+      if (i >= M.m) continue;
+      const int8_t t = bt[u];
+      double bd;
+      if (a[u] > PRIMAL_PIV && t != BT_UPPER && t != BT_FREE) bd = lb[u];
+      else if (a[u] < -PRIMAL_PIV && (t == BT_UPPER || t == BT_FIXED || t == BT_BOXED)) bd = ub[u];
+      else continue;
+      if ((x[u] - bd) / a[u] <= g) argmax_merge(best, bi, fabs(a[u]), i);
+    }
+  }
+}
 // The leaving row fi (INT_MAX: none, ST_NO_PIVOT) is recorded; with BTRAN the
 // workgroup goes on with BTRAN's set-up for it, as k_price<true> does.  Every
 // thread of the workgroup calls it.
@@ -1575,7 +1611,7 @@ __global__ __launch_bounds__(BS) void k_pratio_btran(DevModel M, DevIter I, int 
   if (!(g < BIG)) return;
   double best = -1.0;
   int bi = INT_MAX;
-  pratio2_scan(M, I, dir, g, threadIdx.x, BS, best, bi);
+  pratio2_scan_wg(M, I, dir, g, best, bi);
   block_argmax<BS>(best, bi);
   pratio_leave<true>(M, I, it, F, t, dir, bi);
 }
