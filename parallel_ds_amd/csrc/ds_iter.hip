@@ -679,14 +679,17 @@ __global__ __launch_bounds__(BS) void k_chuzc_tb(DevModel M, DevIter I, int phas
   chuzc_tb_body(M, I, phase, it, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase, int harris, int it) {
+// bid, nb: this workgroup and the number taking part (all of k_chuzc1's grid,
+// or the CHUZC share of a fused stage launch).
+__device__ __forceinline__ void chuzc1_body(const DevModel& M, const DevIter& I, int phase, int harris, int it,
+                                            int bid, int nb) {
   DevScalars* S = I.S;
   if (stopped(S)) return;
   const int neg = S->neg;
   const int nN = M.N - M.m;
   double mn = BIG;
   int nF = 0;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < nN; i += gridDim.x * BS) {
+  for (int i = bid * BS + threadIdx.x; i < nN; i += nb * BS) {
     double a = I.alpha_r[i];
     if (neg) a = -a;
     int j = I.nonbasis[i];
@@ -699,13 +702,13 @@ __global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase,
   mn = block_min<BS>(mn);
   nF = block_sumi<BS>(nF);
   if (threadIdx.x == 0) {
-    I.part_d[blockIdx.x] = mn;
-    I.part_i[blockIdx.x] = nF;
+    I.part_d[bid] = mn;
+    I.part_i[bid] = nF;
   }
-  if (!last_block(&S->ctr[CTR_CHUZC1])) return;
+  if (!last_block(&S->ctr[CTR_CHUZC1], nb)) return;
   double g = BIG;
   int tot = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
+  for (int b = threadIdx.x; b < nb; b += BS) {
     g = fmin(g, I.part_d[b]);
     tot += I.part_i[b];
   }
@@ -717,6 +720,9 @@ __global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase,
     S->tmp0 = g;
     if (tot == 0) raise_status(S, ST_NO_RATIO, it);
   }
+}
+__global__ __launch_bounds__(BS) void k_chuzc1(DevModel M, DevIter I, int phase, int harris, int it) {
+  chuzc1_body(M, I, phase, harris, it, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BS) void k_chuzc2(DevModel M, DevIter I, int phase, int harris, int it) {
@@ -1059,6 +1065,13 @@ __global__ __launch_bounds__(BS) void k_stage_spmv_gemv(DevModel M, const int* _
 __global__ __launch_bounds__(BS) void k_stage_chuzc_slack(DevModel M, DevIter I, int phase, int it, int nb1,
                                                           DevFactor F, const double* rho, double* tau0) {
   if ((int)blockIdx.x < nb1) chuzc_tb_body(M, I, phase, it, blockIdx.x, nb1);
+  else ftran_slack_body(M, F, rho, tau0, I.S, 1, blockIdx.x - nb1);
+}
+
+// Harris: pass 1 of CHUZC beside tau's slack rows (pass 2 follows in k_chuzc2).
+__global__ __launch_bounds__(BS) void k_stage_chuzc1_slack(DevModel M, DevIter I, int phase, int it, int nb1,
+                                                           DevFactor F, const double* rho, double* tau0) {
+  if ((int)blockIdx.x < nb1) chuzc1_body(M, I, phase, 1, it, blockIdx.x, nb1);
   else ftran_slack_body(M, F, rho, tau0, I.S, 1, blockIdx.x - nb1);
 }
 
@@ -2294,9 +2307,14 @@ void launch_stage_spmv_gemv(hipStream_t s, const DevModel& M, const DevIter& I, 
   k_stage_spmv_gemv<<<nb1 + nb2, BS, 0, s>>>(M, posN, rho, cvec, alpha_r, I.S, cols, nb1, F, tau0);
 }
 void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase,
-                              int it, const double* rho, double* tau0) {
+                              int it, const double* rho, double* tau0, int harris) {
   const int nb1 = cap_grid(M.N - M.m, BS, I.part_cap);
-  k_stage_chuzc_slack<<<nb1 + nblk(M.m, BS), BS, 0, s>>>(M, I, phase, it, nb1, F, rho, tau0);
+  if (!harris) {
+    k_stage_chuzc_slack<<<nb1 + nblk(M.m, BS), BS, 0, s>>>(M, I, phase, it, nb1, F, rho, tau0);
+    return;
+  }
+  k_stage_chuzc1_slack<<<nb1 + nblk(M.m, BS), BS, 0, s>>>(M, I, phase, it, nb1, F, rho, tau0);
+  k_chuzc2<<<nb1, BS, 0, s>>>(M, I, phase, 1, it);
 }
 void launch_stage_col_out(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t,
                           double* colout, const double* tau0, double* tau) {

@@ -736,16 +736,16 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
   // execution interval (hipExtLaunchKernelGGL).
   // Every time_kernels-th iteration: a per-launch event pair costs the loop a few percent.
   const bool timed = P_.time_kernels > 0 && marked_ && it % P_.time_kernels == 0 && ev_used_ < (int)ev_iter_.size();
-  if (!timed && !P_.profile && P_.ratio_test == 0 && !has_boxed_) {
+  if (!timed && !P_.profile && (P_.ratio_test == 0 || P_.ratio_test == 1) && !has_boxed_) {
     // CHUZR goes on with BTRAN's set-up, and the DSE FTRAN's stages share the
     // launches of the SpMV, CHUZC and the entering column's FTRAN (same
     // kernels' code, same results), and up to 16,384 update rows the entering
     // column's FTRAN and both sets of etas run in the update's launch: five
-    // launches per iteration.
+    // launches per iteration (six with the Harris ratio test's second pass).
     launch_price_btran(st_, m, i, f, phase, it, skip_top, P_.perturb, t, rho_.p);
     double* tau0 = t > 0 ? w2_.p : tau_.p;
     launch_stage_spmv_gemv(st_, m, i, f, posN_.p, rho_.p, c_.p, alpha_r_.p, tau0);
-    launch_stage_chuzc_slack(st_, m, i, f, phase, it, rho_.p, tau0);
+    launch_stage_chuzc_slack(st_, m, i, f, phase, it, rho_.p, tau0, P_.ratio_test);
     if (phase == 2 && force_due(DS_FORCE_STABILITY, it)) {
       launch_force(st_, m, i, DS_FORCE_STABILITY);
     }
