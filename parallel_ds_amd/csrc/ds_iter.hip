@@ -963,8 +963,8 @@ __global__ __launch_bounds__(BS) void k_ftran_out(DevModel M, DevIter I, const d
 // computes x0 = B0^{-1} a at the etas' pivots itself (the same arithmetic as
 // k_ftran_col), forms the eta coefficients, then writes x0 and the result for
 // its own rows.  Results are bit-for-bit those of k_ftran_col + k_ftran_out.
-__global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, DevFactor F, int col, double* out,
-                                                      int t, int check) {
+__device__ __forceinline__ void ftran_col_eta_body(const DevModel& M, const DevIter& I, const DevFactor& F, int col,
+                                                   double* out, int t, int check, int bid) {
   DevScalars* S = I.S;
   if (skip_launch(S, check)) return;
   const int j = col >= 0 ? col : S->q;
@@ -975,7 +975,7 @@ __global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, Dev
   __shared__ double sk[ETA_MAX];
   __shared__ int pk[ETA_MAX];
   extern __shared__ double mft[];
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = bid * BS + threadIdx.x;
   const bool live = i < M.m;
   const ColRow w = live ? col_row(F, i) : ColRow{0, 0, 0, 0};
   const int ne = load_col(M, F, j, er, eri, ev);
@@ -992,6 +992,17 @@ __global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, Dev
   const double x0 = ftran_col_value(F, w, ne, er, eri, ev);
   I.x0[i] = x0;
   out[i] = eta_apply<64>(x0, I.etas + i, M.m, sk, pk, i, t);
+}
+__global__ __launch_bounds__(BS) void k_ftran_col_eta(DevModel M, DevIter I, DevFactor F, int col, double* out,
+                                                      int t, int check) {
+  ftran_col_eta_body(M, I, F, col, out, t, check, blockIdx.x);
+}
+// The entering column's FTRAN with its etas (x0 and alpha_q), beside tau's
+// etas (tau0 -> tau) in workgroups [nb1, ...).
+__global__ __launch_bounds__(BS) void k_stage_col_eta_out(DevModel M, DevIter I, DevFactor F, int nb1,
+                                                          const double* tau0, double* tau, int t) {
+  if ((int)blockIdx.x < nb1) ftran_col_eta_body(M, I, F, -1, I.alpha_q, t, 1, blockIdx.x);
+  else ftran_out_body(M, I, tau0, tau, t, 0, 1, blockIdx.x - nb1);
 }
 
 // Dense FTRAN pieces: gather b_R, y = K^{-1} b_R (one wave per row), slack rows.
@@ -2379,8 +2390,8 @@ void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const
                        int t, const double* tau0) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
   if (n > UPDATE_ETA_MAX_N) {
-    launch_stage_col_out(s, M, I, F, t, t > 0 ? I.x0 : I.alpha_q, tau0, I.tau);
-    launch_ftran_eta(s, M, I, I.x0, I.alpha_q, t);
+    const int nb1 = nblk(M.m, BS);
+    k_stage_col_eta_out<<<nb1 + (t > 0 ? nb1 : 0), BS, mf_lds_bytes(t), s>>>(M, I, F, nb1, tau0, I.tau, t);
     launch_dupdate_update(s, M, I, phase, it);
     return;
   }

@@ -44,8 +44,8 @@ void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 // The rest of a fused dual iteration after CHUZC: the entering column's FTRAN
 // with its etas, tau's etas (tau0 -> tau), the dual update and the basis
-// change; one launch up to 16,384 update rows, else launch_stage_col_out +
-// launch_ftran_eta + launch_dupdate_update.
+// change; one launch up to 16,384 update rows, else two (the FTRANs, then
+// launch_dupdate_update).
 void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
                        int t, const double* tau0);
 // Primal simplex iterations of a warm start: entering column (Devex), the
