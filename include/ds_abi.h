@@ -101,6 +101,10 @@ typedef struct {
                              simplex starts from the given basis directly */
   double warm_shift;      /* relative bound shift of warm_primal; default 1e-3 */
   int warm_primal_cap;    /* most primal iterations before the dual simplex takes over; 0 = 4 m */
+  int split_launches;     /* test hook, 0 = off: 1 runs the iteration stages that share one launch below
+                             a size limit (the ratio test's two passes, m <= 8,192; the entering
+                             column's FTRAN with the dual update, n <= 16,384) in the launches larger
+                             LPs take.  The results are the same bit for bit */
 } ds_params;
 
 /* Recovery paths a test can force from the host (sequentialDualSimplex.cpp):

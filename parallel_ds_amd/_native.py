@@ -21,7 +21,8 @@ class Params(C.Structure):
                 ("mark_iteration", C.c_int), ("time_kernels", C.c_int), ("cleanup", C.c_int),
                 ("profile", C.c_int), ("force_event", C.c_int), ("force_at", C.c_int),
                 ("time_limit_ms", C.c_double), ("candidates", C.c_int), ("candidate_psi", C.c_double),
-                ("warm_primal", C.c_int), ("warm_shift", C.c_double), ("warm_primal_cap", C.c_int)]
+                ("warm_primal", C.c_int), ("warm_shift", C.c_double), ("warm_primal_cap", C.c_int),
+                ("split_launches", C.c_int)]
 
 
 DS_FORCE_NONE, DS_FORCE_STABILITY, DS_FORCE_DUAL_RESTART, DS_FORCE_SINGULAR = 0, 1, 2, 3

@@ -110,6 +110,7 @@ void ds_default_params(ds_params* p) {
   p->warm_primal = 0;
   p->warm_shift = 1e-3;
   p->warm_primal_cap = 0;
+  p->split_launches = 0;
 }
 
 int ds_create(int device, ds_ctx** out) {

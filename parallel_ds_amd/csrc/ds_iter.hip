@@ -2387,9 +2387,9 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
 // (each reads MF's triangle) cost more than the launches they save.
 constexpr int UPDATE_ETA_MAX_N = 16384;
 void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
-                       int t, const double* tau0) {
+                       int t, const double* tau0, int split) {
   const int n = (M.N - M.m) > M.m ? (M.N - M.m) : M.m;
-  if (n > UPDATE_ETA_MAX_N) {
+  if (split || n > UPDATE_ETA_MAX_N) {
     const int nb1 = nblk(M.m, BS);
     k_stage_col_eta_out<<<nb1 + (t > 0 ? nb1 : 0), BS, mf_lds_bytes(t), s>>>(M, I, F, nb1, tau0, I.tau, t);
     launch_dupdate_update(s, M, I, phase, it);
@@ -2401,9 +2401,9 @@ void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it) {
   k_pprice<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, it);
 }
 void launch_pratio_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int it, int t,
-                         double* rho) {
+                         double* rho, int split) {
   const int g = cap_grid(M.m, BS, I.part_cap);
-  if (M.m <= PRATIO_ONE_WG) {
+  if (!split && M.m <= PRATIO_ONE_WG) {
     k_pratio_btran<<<g, BS, mf_lds_bytes(t), s>>>(M, I, it, F, t);
   } else {
     k_pratio1<<<g, BS, 0, s>>>(M, I, it);

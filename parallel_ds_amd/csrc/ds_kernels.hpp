@@ -46,16 +46,18 @@ void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, i
 // with its etas, tau's etas (tau0 -> tau), the dual update and the basis
 // change; one launch up to 16,384 update rows, else two (the FTRANs, then
 // launch_dupdate_update).
+// split = 1: the two-launch form at any size (test hook).
 void launch_col_update(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase, int it,
-                       int t, const double* tau0);
+                       int t, const double* tau0, int split = 0);
 // Primal simplex iterations of a warm start: entering column (Devex), the
 // two-pass ratio test over the basic rows, and the update (after FTRAN, BTRAN
 // and the pivot-row SpMV).
 void launch_pprice(hipStream_t s, const DevModel& M, const DevIter& I, int it);
 // The primal ratio test's two passes, the second one going on with BTRAN's
 // set-up for the leaving row, then BTRAN's output (rho).
+// split = 1: the two passes in launches of their own at any size (test hook).
 void launch_pratio_btran(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int it, int t,
-                         double* rho);
+                         double* rho, int split = 0);
 void launch_pupdate(hipStream_t s, const DevModel& M, const DevIter& I, int it);
 // Bound-flipping ratio test: the flip loop after CHUZC, then the flip set, the
 // column change, its FTRAN and the x_B / objective correction.

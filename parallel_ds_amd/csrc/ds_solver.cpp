@@ -749,7 +749,7 @@ void Solver::enqueue_body(int phase, int it, int skip_top, bool dualcheck, int t
     if (phase == 2 && force_due(DS_FORCE_STABILITY, it)) {
       launch_force(st_, m, i, DS_FORCE_STABILITY);
     }
-    launch_col_update(st_, m, i, f, phase, it, t, tau0);
+    launch_col_update(st_, m, i, f, phase, it, t, tau0, P_.split_launches);
     return;
   }
   launch_price(st_, m, i, phase, it, skip_top, P_.perturb);
@@ -1195,7 +1195,7 @@ void Solver::enqueue_primal(int it, int t) {
   const DevFactor f = F();
   launch_pprice(st_, m, i, it);
   launch_ftran_col(st_, m, i, f, -1, t, alpha_q_.p, 1);
-  launch_pratio_btran(st_, m, i, f, it, t, rho_.p);
+  launch_pratio_btran(st_, m, i, f, it, t, rho_.p, P_.split_launches);
   launch_spmv(st_, m, posN_.p, rho_.p, 1.0, 0.0, c_.p, alpha_r_.p, 0, S_.p, 1);
   launch_pupdate(st_, m, i, it);
 }

@@ -74,3 +74,24 @@ def test_models_with_boxed_variables_skip_the_primal_phase(gpu):
     sol, basis, r = solve_lp(L, default_params(warm_primal=1, max_iterations=20000), start=b0)
     assert r.status == DS_OPTIMAL and r.primal_iterations == 0
     assert abs(sol.Z - r0.Z) <= 1e-9 * max(1.0, abs(r0.Z))
+
+
+def test_split_launches_give_the_same_solves(gpu):
+    """Below a size limit, stages share launches: both ratio-test passes with BTRAN's set-up
+    (m <= 8,192), and the entering column's FTRAN, its etas, tau's etas and the dual update
+    (n <= 16,384).  params.split_launches runs them in the launches larger LPs take; the solves
+    must be the same bit for bit: status, iterations, objective and final basis.  Warm starts with
+    primal iterations on the m = 256 family, and slack starts with both ratio tests."""
+    base = lpmod.gen_kcol(256, 640, 5, 4)
+    _, start, r0 = solve_lp(lpmod.perturbed_instance(base, 0.1, 10000), default_params())
+    assert r0.status == DS_OPTIMAL
+    cases = [(lpmod.perturbed_instance(base, 0.1, s), start, dict(warm_primal=1)) for s in (20000, 20003)]
+    cases += [(lpmod.config0(1), None, dict(ratio_test=rt)) for rt in (0, 1)]
+    for L, st, kw in cases:
+        outs = []
+        for split in (0, 1):
+            sol, basis, r = solve_lp(L, default_params(max_iterations=50000, split_launches=split, **kw), start=st)
+            outs.append((r.status, r.iterations, r.primal_iterations, sol.Z, sorted(basis.tolist())))
+        print(kw, "iterations", outs[0][1], outs[1][1])
+        assert outs[0][0] == DS_OPTIMAL
+        assert outs[0] == outs[1]
