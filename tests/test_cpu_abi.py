@@ -82,3 +82,36 @@ def test_generator_shapes():
     assert np.all(T.costs >= 1) and np.all(T.costs <= 1000)
     B = lpmod.config3_instance(5)
     assert 32 <= B.m <= 256 and B.n == 2 * B.m and np.all(np.diff(B.col_ptr) >= 1)
+
+
+def test_python_struct_mirrors_match_the_c_layout():
+    """The ctypes mirrors of ds_params, ds_result and ds_batch_params have the C compiler's size
+    and the offset of every field (compiled with gcc against include/ds_abi.h)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import pytest
+
+    from parallel_ds_amd._native import BatchParams, Params, Result
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    mirrors = [("ds_params", Params), ("ds_result", Result), ("ds_batch_params", BatchParams)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ds_abi.h"', "int main(void) {"]
+    for cname, py in mirrors:
+        lines.append('  printf("%%s %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('  printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append("  return 0;\n}\n")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(l.split() for l in out if l)
+    for cname, py in mirrors:
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got["%s.%s" % (cname, f[0])]) == getattr(py, f[0]).offset, (cname, f[0])
