@@ -8,8 +8,9 @@ trained on the optimal bases the GPU solver finds for --train instances; a step 
 instance solved on the single-LP GPU path (--threads at a time, one HIP stream each).  --warmup W
 held-out instances run untimed, then --steps K instances are solved from the slack basis and from
 the learned warm start; `value` is the learned-warm solves/s over those K, with the slack-start
-solves/s, iterations and objective agreement beside it.  Inputs are resident in HBM before each
-solve starts; the wall time includes the host's per-chunk synchronisation.
+solves/s, iterations and objective agreement beside it.  Each solving thread creates its context
+and uploads its LP (about 1 MB) inside the timed wall, which also includes the host's per-chunk
+synchronisation.
 
 Also reported: the single large LP (m=20,000, n=40,000, 10 nonzeros per column, refactor every
 64) over a fixed window of iterations 501..2,500 with the pivot-row SpMV roofline (`roofline`,
