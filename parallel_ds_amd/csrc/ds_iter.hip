@@ -1086,12 +1086,6 @@ __global__ __launch_bounds__(BS) void k_stage_chuzc1_slack(DevModel M, DevIter I
   else ftran_slack_body(M, F, rho, tau0, I.S, 1, blockIdx.x - nb1);
 }
 
-__global__ __launch_bounds__(BS) void k_stage_col_out(DevModel M, DevIter I, DevFactor F, double* colout, int nb1,
-                                                      const double* tau0, double* tau, int t) {
-  if ((int)blockIdx.x < nb1) ftran_col_body(M, I, F, -1, colout, 1, blockIdx.x);
-  else ftran_out_body(M, I, tau0, tau, t, 0, 1, blockIdx.x - nb1);
-}
-
 // ---------------------------------------------------------------------------
 // Reduced-cost update (sequentialDualSimplex.cpp:428-467, and
 // simpleReducedCostsUpate :163-172 in phase 1) with the stability check.
@@ -1129,23 +1123,6 @@ __device__ __forceinline__ void dupdate_final(const DevModel& M, const DevIter& 
     S->obj = S->obj - theta * S->f_p;
   }
   S->alpha_p = aqp;
-}
-
-__global__ __launch_bounds__(BS) void k_dupdate(DevModel M, DevIter I, int phase, int it) {
-  DevScalars* S = I.S;
-  if (stopped(S)) return;
-  const double theta = S->theta;
-  int dinf = 0;
-  dupdate_body(M, I, phase, theta, S->q_idx, dinf);
-  dinf = block_sumi<BS>(dinf);
-  if (threadIdx.x == 0) I.part_i[blockIdx.x] = dinf;
-  if (!last_block(&S->ctr[CTR_DUPD])) return;
-  int td = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) td += I.part_i[b];
-  td = block_sumi<BS>(td);
-  if (threadIdx.x != 0) return;
-  S->ctr[CTR_DUPD] = 0;
-  dupdate_final(M, I, phase, it, theta, td, I.alpha_q[S->p_idx]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1307,8 +1284,9 @@ __global__ __launch_bounds__(BS) void k_dupdate_update(DevModel M, DevIter I, in
 // updates them.  The last workgroup takes alpha_q at the older etas' pivots
 // from the E table (E[k][j] holds eta k at eta j's pivot) in the same order
 // of operations, so that it reads no entry another workgroup of this launch
-// wrote.  Results are bit-for-bit those of k_stage_col_out, k_ftran_out and
-// k_dupdate_update.  tau0 is tau itself when t == 0.
+// wrote.  Results are bit-for-bit those of the separate launches
+// (k_stage_col_eta_out, then k_dupdate_update).  tau0 is tau itself when
+// t == 0.
 __global__ __launch_bounds__(BS) void k_dupdate_update_col(DevModel M, DevIter I, DevFactor F, int phase, int it,
                                                            const double* tau0, int t) {
   DevScalars* S = I.S;
@@ -2327,15 +2305,6 @@ void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I
   k_stage_chuzc1_slack<<<nb1 + nblk(M.m, BS), BS, 0, s>>>(M, I, phase, it, nb1, F, rho, tau0);
   k_chuzc2<<<nb1, BS, 0, s>>>(M, I, phase, 1, it);
 }
-void launch_stage_col_out(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t,
-                          double* colout, const double* tau0, double* tau) {
-  const int nb1 = nblk(M.m, BS);
-  k_stage_col_out<<<nb1 + (t > 0 ? nblk(M.m, BS) : 0), BS, mf_lds_bytes(t), s>>>(M, I, F, colout, nb1, tau0, tau,
-                                                                                  t);
-}
-void launch_ftran_eta(hipStream_t s, const DevModel& M, const DevIter& I, const double* x0, double* out, int t) {
-  if (t > 0) k_ftran_out<<<nblk(M.m, BS), BS, mf_lds_bytes(t), s>>>(M, I, x0, out, t, 0, 1);
-}
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it) {
   int g = cap_grid(M.N - M.m, BS, I.part_cap);
   if (!harris) {
@@ -2372,9 +2341,6 @@ void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, cons
     k_gemv<<<nblk((long)F.k * WAVE, BS), BS, 0, s>>>(F.KinvT, F.k, F.k, I.wN, nullptr, y, F.row_R, I.S, 0);
   }
   k_btran_dense_slack<<<nblk(M.m, BS), BS, 0, s>>>(M, F, v, y);
-}
-void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
-  k_dupdate<<<cap_grid(M.N - M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);
 }
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it) {
   k_update<<<cap_grid(M.m, BS, I.part_cap), BS, 0, s>>>(M, I, phase, it);

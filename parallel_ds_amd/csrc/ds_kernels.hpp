@@ -21,25 +21,19 @@ void launch_spmv(hipStream_t s, const DevModel& M, const int* posN, const double
 void launch_chuzc(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int harris, int it);
 // Independent stages of one iteration in shared launches (no bound flips):
 // pivot-row SpMV + tau's kernel product into tau0; CHUZC + tau's slack rows;
-// the entering column's FTRAN (into colout: x0 when t > 0, else alpha_q) +
-// tau's etas (tau0 -> tau); then the column's etas (launch_ftran_eta, x0 ->
-// alpha_q); or all of the last two inside launch_col_update.  tau0 is tau
-// itself when t == 0.
+// then launch_col_update (the entering column's FTRAN with its etas, tau's
+// etas, the updates).  tau0 is tau itself when t == 0.
 void launch_stage_spmv_gemv(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const int* posN,
                             const double* rho, const double* cvec, double* alpha_r, double* tau0);
 // harris = 1: pass 1 beside the slack rows, then pass 2 in a launch of its own.
 void launch_stage_chuzc_slack(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int phase,
                               int it, const double* rho, double* tau0, int harris = 0);
-void launch_stage_col_out(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int t,
-                          double* colout, const double* tau0, double* tau);
-void launch_ftran_eta(hipStream_t s, const DevModel& M, const DevIter& I, const double* x0, double* out, int t);
 void launch_ftran_col(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, int col, int t,
                       double* out, int check);
 void launch_ftran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, const double* b,
                         double* out, int t, int mode, int check);
 void launch_btran_dense(hipStream_t s, const DevModel& M, const DevIter& I, const DevFactor& F, double* v,
                         double* y, int t);
-void launch_dupdate(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 void launch_dupdate_update(hipStream_t s, const DevModel& M, const DevIter& I, int phase, int it);
 // The rest of a fused dual iteration after CHUZC: the entering column's FTRAN
