@@ -1234,6 +1234,8 @@ void Solver::primal_phase() {
         dualSimplexInit();
         shifted_ += shift_bounds();
       }
+      // A new Devex reference framework (all weights 1) at every refactor.
+      launch_fill(st_, devex_.p, 1.0, H_.N);
     }
     int L = 0;
     for (int i = next;; ++i) {
